@@ -1,0 +1,413 @@
+/*
+ * cube_ctx.hip -- context lifetime, state transfer and the step entry points
+ * of the C ABI (include/cube.h).
+ */
+#include "cube_internal.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <new>
+
+extern "C" const char *cube_strerror(int code) {
+  switch (code) {
+    case CUBE_OK: return "ok";
+    case CUBE_ERR_TILE_OVERFLOW: return "too many particles in a tile+buffer: set tile_buffer larger";
+    case CUBE_ERR_IMAGE_OVERFLOW: return "too many particles in the image+buffer: set image_buffer larger";
+    case CUBE_ERR_PP_OVERFLOW: return "np_pp_max too small";
+    case CUBE_ERR_CHECKSUM: return "particle checksum mismatch";
+    case CUBE_ERR_COUNT: return "particle count mismatch";
+    case CUBE_ERR_DISPLACEMENT: return "particle displacement larger than the buffer";
+    case CUBE_ERR_HIP: return "HIP runtime error";
+    case CUBE_ERR_ALLOC: return "device allocation failed";
+    case CUBE_ERR_INVALID: return "invalid argument or unsupported size";
+    case CUBE_ERR_STATE: return "call out of order";
+  }
+  return "unknown error";
+}
+
+static void free_all(cube_ctx *c) {
+  void *ptrs[] = {c->xp, c->vp, c->xp2, c->vp2, c->pid, c->pid2, c->rhoc, c->rhoc2, c->vfield, c->vfield2,
+                  c->cstart, c->cstart2, c->dest, c->key, c->sortbuf, c->fill, c->xdest, c->xkey,
+                  c->tanf_tab, c->vdec0, c->vdec1, c->rho_f, c->spec_f, c->specd_f, c->force_f,
+                  c->kern_f, c->rho_c, c->spec_c, c->specd_c, c->force_c, c->kern_c, c->dred,
+                  c->ured, c->ired, c->scanbuf};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  cube_fft_destroy(c->plan_f);
+  cube_fft_destroy(c->plan_c);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+template <typename T> static int dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return CUBE_ERR_ALLOC;
+  }
+  return CUBE_OK;
+}
+static int dalloc_bytes(void **p, size_t n) {
+  if (hipMalloc(p, n ? n : 1) != hipSuccess) {
+    *p = nullptr;
+    return CUBE_ERR_ALLOC;
+  }
+  return CUBE_OK;
+}
+
+extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
+  if (!p || !out) return CUBE_ERR_INVALID;
+  *out = nullptr;
+  if (p->nc < 8 || p->nnt < 1 || p->nc % p->nnt || p->ncb < 1 || (p->izipx != 1 && p->izipx != 2) ||
+      (p->izipv != 1 && p->izipv != 2) || p->np_nc < 1 || p->pp_range < 0 || p->pp_range > 4)
+    return CUBE_ERR_INVALID;
+  const int64_t nt = p->nc / p->nnt;
+  if (nt < p->ncb) return CUBE_ERR_INVALID;
+  cube_ctx *c = new (std::nothrow) cube_ctx;
+  if (!c) return CUBE_ERR_ALLOC;
+  memset(c, 0, sizeof(*c));
+  c->p = *p;
+  c->device = p->device;
+  if (hipSetDevice(c->device) != hipSuccess) {
+    delete c;
+    return CUBE_ERR_HIP;
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return CUBE_ERR_HIP;
+  }
+  c->nc = p->nc; c->nnt = p->nnt; c->nt = nt; c->ncb = p->ncb; c->ncell = 4;
+  c->nf = c->nc * c->ncell; c->nh = c->nf / 2 + 1;
+  c->ncells = c->nc * c->nc * c->nc;
+  c->ntiles = c->nnt * c->nnt * c->nnt;
+  c->nvbin = 1 << (8 * p->izipv);
+  /* W/variables.f90:8-10 in real(4), as the reference evaluates them */
+  c->np_image = (c->nc * p->np_nc) * (c->nc * p->np_nc) * (c->nc * p->np_nc);
+  {
+    const float r = (float)(nt + 2 * p->ncb) * 1.0f / (float)nt;
+    const float r3 = r * (r * r);
+    c->np_image_max = (int64_t)((float)c->np_image * r3 * p->image_buffer);
+    c->np_tile_max = (int64_t)((float)(c->np_image / c->ntiles) * r3 * p->tile_buffer);
+    c->np_pp_max = c->np_tile_max;
+  }
+  /* the disjoint state never holds buffer copies: the total particle count
+     is conserved, so capacity only needs headroom for the rare duplicates */
+  c->cap = c->np_image + c->np_image / 64 + 4096;
+  c->xcap = c->np_image / 64 + 4096;
+  int st = CUBE_OK;
+  const size_t xb = (size_t)p->izipx * 3, vb = (size_t)p->izipv * 3;
+  st |= dalloc_bytes(&c->xp, c->cap * xb);
+  st |= dalloc_bytes(&c->vp, c->cap * vb);
+  st |= dalloc_bytes(&c->xp2, c->cap * xb);
+  st |= dalloc_bytes(&c->vp2, c->cap * vb);
+  if (p->has_pid) {
+    st |= dalloc(&c->pid, c->cap);
+    st |= dalloc(&c->pid2, c->cap);
+  }
+  st |= dalloc(&c->rhoc, c->ncells);
+  st |= dalloc(&c->rhoc2, c->ncells);
+  st |= dalloc(&c->vfield, 3 * c->ncells);
+  st |= dalloc(&c->vfield2, 3 * c->ncells);
+  st |= dalloc(&c->cstart, c->ncells + 1);
+  st |= dalloc(&c->cstart2, c->ncells + 1);
+  st |= dalloc(&c->dest, c->cap);
+  st |= dalloc(&c->key, c->cap);
+  st |= dalloc(&c->sortbuf, c->cap + c->xcap);
+  st |= dalloc(&c->fill, c->ncells);
+  st |= dalloc(&c->xdest, c->xcap);
+  st |= dalloc(&c->xkey, c->xcap);
+  st |= dalloc(&c->tanf_tab, c->nvbin);
+  st |= dalloc(&c->vdec0, c->nvbin);
+  st |= dalloc(&c->vdec1, c->nvbin);
+  const int64_t nf3 = c->nf * c->nf * c->nf, nspec = c->nh * c->nf * c->nf;
+  const int64_t ho = c->nf / 2 + 1;
+  st |= dalloc(&c->rho_f, nf3);
+  st |= dalloc(&c->spec_f, nspec);
+  st |= dalloc(&c->specd_f, 3 * nspec);
+  st |= dalloc(&c->force_f, 3 * nf3);
+  st |= dalloc(&c->kern_f, 3 * ho * ho * ho);
+  const int64_t nc3 = c->ncells, nspec_c = (c->nc / 2 + 1) * c->nc * c->nc;
+  st |= dalloc(&c->rho_c, nc3);
+  st |= dalloc(&c->spec_c, nspec_c);
+  st |= dalloc(&c->specd_c, 3 * nspec_c);
+  st |= dalloc(&c->force_c, 3 * nc3);
+  st |= dalloc(&c->kern_c, 3 * nspec_c);
+  st |= dalloc(&c->dred, 4096);
+  st |= dalloc(&c->ured, 64);
+  st |= dalloc(&c->ired, 4096);
+  st |= dalloc(&c->scanbuf, c->ncells / 1024 + 4);
+  if (st) {
+    free_all(c);
+    delete c;
+    return CUBE_ERR_ALLOC;
+  }
+  if ((st = cube_fft_create(c, (int)c->nf, &c->plan_f)) || (st = cube_fft_create(c, (int)c->nc, &c->plan_c))) {
+    free_all(c);
+    delete c;
+    return st;
+  }
+  /* tan table: tan(pi*real(vp)/real(nvbin-1)) in real(4), computed with the
+     host libm tanf exactly as the reference's compiled Fortran calls it
+     (W/update_particle.f90:67) */
+  {
+    const int nv = c->nvbin;
+    float *h = (float *)malloc(nv * sizeof(float));
+    const float nvm1 = (float)(nv - 1);
+    for (int i = 0; i < nv; ++i) {
+      const int code = i - nv / 2;
+      h[i] = tanf((CUBE_PI_F * (float)code) / nvm1);
+    }
+    const hipError_t e = hipMemcpy(c->tanf_tab, h, nv * sizeof(float), hipMemcpyHostToDevice);
+    free(h);
+    if (e != hipSuccess) {
+      free_all(c);
+      delete c;
+      return CUBE_ERR_HIP;
+    }
+  }
+  c->pp_ftemp = 0.0f;
+  *out = c;
+  return CUBE_OK;
+}
+
+extern "C" void cube_ctx_destroy(cube_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+}
+
+extern "C" int cube_ctx_info(const cube_ctx *c, int64_t *o) {
+  if (!c || !o) return CUBE_ERR_INVALID;
+  o[0] = c->nt; o[1] = c->nf; o[2] = c->nh; o[3] = c->np_image; o[4] = c->np_image_max;
+  o[5] = c->np_tile_max; o[6] = c->np_pp_max; o[7] = c->cap; o[8] = c->nplocal; o[9] = c->npglobal;
+  o[10] = c->has_kernels; o[11] = c->has_state; o[12] = c->nc; o[13] = c->nnt; o[14] = c->ncb;
+  o[15] = c->nvbin;
+  return CUBE_OK;
+}
+
+extern "C" int cube_sync(cube_ctx *c) {
+  if (!c) return CUBE_ERR_INVALID;
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  CUBE_CHECK(hipGetLastError());
+  return CUBE_OK;
+}
+
+extern "C" double cube_host_atan(double x) { return cube_atan(x); }
+
+/* ---- state transfer ---------------------------------------------------- */
+extern "C" int cube_set_state(cube_ctx *c, int64_t nplocal, int64_t npglobal, const void *xp, const void *vp,
+                              const int32_t *pid, const int32_t *rhoc, const float *vfield) {
+  if (!c || !xp || !vp || !rhoc || !vfield || nplocal < 0) return CUBE_ERR_INVALID;
+  if (nplocal > c->cap) return CUBE_ERR_IMAGE_OVERFLOW;
+  CUBE_CHECK(hipSetDevice(c->device));
+  int64_t s = 0;
+  for (int64_t q = 0; q < c->ncells; ++q) s += rhoc[q];
+  if (s != nplocal) return CUBE_ERR_COUNT;
+  CUBE_CHECK(hipMemcpy(c->xp, xp, nplocal * 3 * c->p.izipx, hipMemcpyHostToDevice));
+  CUBE_CHECK(hipMemcpy(c->vp, vp, nplocal * 3 * c->p.izipv, hipMemcpyHostToDevice));
+  if (c->p.has_pid) {
+    if (pid) {
+      CUBE_CHECK(hipMemcpy(c->pid, pid, nplocal * sizeof(int32_t), hipMemcpyHostToDevice));
+    } else {
+      CUBE_CHECK(hipMemset(c->pid, 0, nplocal * sizeof(int32_t)));
+    }
+  }
+  CUBE_CHECK(hipMemcpy(c->rhoc, rhoc, c->ncells * sizeof(int32_t), hipMemcpyHostToDevice));
+  CUBE_CHECK(hipMemcpy(c->vfield, vfield, 3 * c->ncells * sizeof(float), hipMemcpyHostToDevice));
+  int64_t tot = 0;
+  int st = cube_scan_counts(c, c->rhoc, c->cstart, c->ncells, &tot);
+  if (st) return st;
+  c->nplocal = nplocal;
+  c->npglobal = npglobal;
+  c->has_state = 1;
+  return CUBE_OK;
+}
+
+extern "C" int cube_get_state(cube_ctx *c, int64_t *nplocal, void *xp, void *vp, int32_t *pid, int32_t *rhoc,
+                              float *vfield) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  const int64_t n = c->nplocal;
+  if (nplocal) *nplocal = n;
+  if (xp) CUBE_CHECK(hipMemcpy(xp, c->xp, n * 3 * c->p.izipx, hipMemcpyDeviceToHost));
+  if (vp) CUBE_CHECK(hipMemcpy(vp, c->vp, n * 3 * c->p.izipv, hipMemcpyDeviceToHost));
+  if (pid && c->p.has_pid) CUBE_CHECK(hipMemcpy(pid, c->pid, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (rhoc) CUBE_CHECK(hipMemcpy(rhoc, c->rhoc, c->ncells * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (vfield) CUBE_CHECK(hipMemcpy(vfield, c->vfield, 3 * c->ncells * sizeof(float), hipMemcpyDeviceToHost));
+  return CUBE_OK;
+}
+
+/* ---- kernels ----------------------------------------------------------- */
+extern "C" int cube_init_kernels(cube_ctx *c, const float *wfxyzf, const float *wfxyzc) {
+  if (!c || !wfxyzf || !wfxyzc) return CUBE_ERR_INVALID;
+  CUBE_CHECK(hipSetDevice(c->device));
+  int st = cube_green_init(c, wfxyzf, wfxyzc);
+  if (st) return st;
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  c->has_kernels = 1;
+  return CUBE_OK;
+}
+
+/* ---- the step ------------------------------------------------------------ */
+extern "C" int cube_update_x(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi,
+                             cube_vstats *out) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  return cube_drift(c, dt_old, dt, vz_max, sigma_vi, out);
+}
+
+/* buffer_grid: the buffered copies are implicit (a buffer cell is read from
+   its periodic neighbour), so what remains of W/buffer_grid.f90 is the
+   image-overflow check of redistribute_cdm (:137-154): the particles of all
+   tiles' extended regions must fit np_image_max. */
+extern "C" int cube_buffer_grid(cube_ctx *c) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  int64_t *sums = (int64_t *)malloc(c->ntiles * sizeof(int64_t));
+  int st = cube_tile_boxsum(c, c->rhoc, (int)c->ncb, sums);
+  int64_t tot = 0;
+  for (int64_t t = 0; t < c->ntiles; ++t) tot += sums[t];
+  free(sums);
+  if (st) return st;
+  if ((double)tot / (double)c->np_image_max > 1.0) return CUBE_ERR_IMAGE_OVERFLOW;
+  return CUBE_OK;
+}
+extern "C" int cube_buffer_x(cube_ctx *c) {
+  if (!c) return CUBE_ERR_INVALID;
+  return c->has_state ? CUBE_OK : CUBE_ERR_STATE;
+}
+extern "C" int cube_buffer_v(cube_ctx *c) {
+  if (!c) return CUBE_ERR_INVALID;
+  return c->has_state ? CUBE_OK : CUBE_ERR_STATE;
+}
+
+extern "C" int cube_pp_force(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, float *f2_max_pp) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  /* W/ext_pp_force.f90:41-45: particles of coarse cells 0..nt+1 of a tile */
+  int64_t *sums = (int64_t *)malloc(c->ntiles * sizeof(int64_t));
+  int st = cube_tile_boxsum(c, c->rhoc, 1, sums);
+  int64_t mx = 0;
+  for (int64_t t = 0; t < c->ntiles; ++t) mx = sums[t] > mx ? sums[t] : mx;
+  free(sums);
+  if (st) return st;
+  if (mx > c->np_pp_max) return CUBE_ERR_PP_OVERFLOW;
+  int64_t itest1 = 0;
+  st = cube_pp(c, a_mid, dt, sigma_vi, mass_p, f2_max_pp, &itest1);
+  if (st) return st;
+  if (itest1 != c->nplocal) return CUBE_ERR_COUNT;
+  return CUBE_OK;
+}
+
+extern "C" int cube_pm_fine_deposit(cube_ctx *c, float mass_p, double *testrhof) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  return cube_fine_deposit(c, mass_p, testrhof);
+}
+extern "C" int cube_pm_fine_force(cube_ctx *c, float *f2) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_kernels) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  int st = cube_fft_r2c_3d(c, c->plan_f, c->rho_f, c->spec_f);
+  if (st) return st;
+  return cube_fft_green_c2r_3d(c, c->plan_f, c->spec_f, c->kern_f, 1, c->specd_f, c->force_f, (float)c->nf, f2);
+}
+extern "C" int cube_pm_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  return cube_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new);
+}
+extern "C" int cube_pm_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  return cube_coarse_deposit(c, mass_p, testrhoc);
+}
+extern "C" int cube_pm_coarse_force(cube_ctx *c, float *f2) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_kernels) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  int st = cube_fft_r2c_3d(c, c->plan_c, c->rho_c, c->spec_c);
+  if (st) return st;
+  return cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc, f2);
+}
+extern "C" int cube_pm_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state) return CUBE_ERR_STATE;
+  CUBE_CHECK(hipSetDevice(c->device));
+  return cube_coarse_kick(c, a_mid, dt, sigma_vi, vmax);
+}
+
+/* particle_mesh, W/pm.f90:2-415 */
+extern "C" int cube_particle_mesh(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new,
+                                  float mass_p, cube_pm_out *out) {
+  if (!c) return CUBE_ERR_INVALID;
+  if (!c->has_state || !c->has_kernels) return CUBE_ERR_STATE;
+  cube_pm_out o;
+  memset(&o, 0, sizeof(o));
+  int st;
+  if ((st = cube_pm_fine_deposit(c, mass_p, &o.testrhof))) return st;
+  if ((st = cube_pm_fine_force(c, &o.f2_max_fine))) return st;
+  if ((st = cube_pm_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new))) return st;
+  sigma_vi = sigma_vi_new; /* W/pm.f90:193 */
+  if ((st = cube_pm_coarse_deposit(c, mass_p, &o.testrhoc))) return st;
+  if ((st = cube_pm_coarse_force(c, &o.f2_max_coarse))) return st;
+  if ((st = cube_pm_coarse_kick(c, a_mid, dt, sigma_vi, o.vmax))) return st;
+  /* W/pm.f90:395-398 in real(4) */
+  const float GG = 1.0f / 6.0f / CUBE_PI_F;
+  o.dt_fine = sqrtf(1.0f / (sqrtf(o.f2_max_fine) * a_mid * GG));
+  o.dt_coarse = sqrtf((float)c->ncell / (sqrtf(o.f2_max_coarse) * a_mid * GG));
+  float vm = o.vmax[0] > o.vmax[1] ? o.vmax[0] : o.vmax[1];
+  vm = vm > o.vmax[2] ? vm : o.vmax[2];
+  o.dt_vmax = 0.9f * 20.0f / vm;
+  o.vz_max = o.vmax[2];
+  if (out) *out = o;
+  return CUBE_OK;
+}
+
+/* ---- fields ---------------------------------------------------------------- */
+static int field_ptr(cube_ctx *c, int which, void **ptr, size_t *bytes) {
+  const int64_t nf3 = c->nf * c->nf * c->nf, ho = c->nf / 2 + 1;
+  switch (which) {
+    case CUBE_FIELD_RHO_F: *ptr = c->rho_f; *bytes = nf3 * 4; return CUBE_OK;
+    case CUBE_FIELD_FORCE_F: *ptr = c->force_f; *bytes = 3 * nf3 * 4; return CUBE_OK;
+    case CUBE_FIELD_RHO_C: *ptr = c->rho_c; *bytes = c->ncells * 4; return CUBE_OK;
+    case CUBE_FIELD_FORCE_C: *ptr = c->force_c; *bytes = 3 * c->ncells * 4; return CUBE_OK;
+    case CUBE_FIELD_KERN_F: *ptr = c->kern_f; *bytes = 3 * ho * ho * ho * 4; return CUBE_OK;
+    case CUBE_FIELD_KERN_C: *ptr = c->kern_c; *bytes = 3 * (c->nc / 2 + 1) * c->nc * c->nc * 4; return CUBE_OK;
+  }
+  return CUBE_ERR_INVALID;
+}
+extern "C" int cube_get_field(cube_ctx *c, int which, void *host, size_t bytes) {
+  if (!c || !host) return CUBE_ERR_INVALID;
+  void *p;
+  size_t n;
+  int st = field_ptr(c, which, &p, &n);
+  if (st) return st;
+  if (bytes != n) return CUBE_ERR_INVALID;
+  CUBE_CHECK(hipSetDevice(c->device));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  CUBE_CHECK(hipMemcpy(host, p, n, hipMemcpyDeviceToHost));
+  return CUBE_OK;
+}
+extern "C" int cube_set_field(cube_ctx *c, int which, const void *host, size_t bytes) {
+  if (!c || !host) return CUBE_ERR_INVALID;
+  void *p;
+  size_t n;
+  int st = field_ptr(c, which, &p, &n);
+  if (st) return st;
+  if (bytes != n) return CUBE_ERR_INVALID;
+  CUBE_CHECK(hipSetDevice(c->device));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  CUBE_CHECK(hipMemcpy(p, host, n, hipMemcpyHostToDevice));
+  if (which == CUBE_FIELD_KERN_F || which == CUBE_FIELD_KERN_C) c->has_kernels = 1;
+  return CUBE_OK;
+}

@@ -1,0 +1,594 @@
+/*
+ * cube_drift.hip -- update_x (W/update_particle.f90:16-256) on the whole box.
+ *
+ * The reference drifts tile by tile over the tile's extended region
+ * (physical + ncb buffer cells, whose particles are copies of neighbouring
+ * tiles), counts particles into new cells (pass 1), builds prefix sums
+ * (spine_tile), and scatters them (pass 2) in the visit order
+ * (ilayer, k, j, i, l) with ilayer = (k-(1-ncb)) mod nlayer; only particles
+ * landing in physical cells survive.  Both passes decode with the same
+ * arithmetic, so a particle's new cell is fixed by the coordinates of the
+ * copy that lands.
+ *
+ * Here there are no buffer copies.  Each particle evaluates, per axis, the
+ * copies the reference would hold of it (home coordinate c, and c-nt in the
+ * next tile if c>nt-ncb, and c+nt in the previous tile if c<=ncb), with the
+ * reference's FP64 expression ceiling(xq+dt_mid*v/ncell) in each copy's own
+ * tile frame, and keeps the copies that land in [1,nt] (normally exactly
+ * one).  Its order inside the destination cell is the 64-bit key
+ *   ilayer(9) | k_src-k_dst+2ncb (5) | j.. (5) | i.. (5) | particle index (40)
+ * which, for a fixed destination cell, sorts exactly like (ilayer,k,j,i,l).
+ * Counting (atomics on integers), an exclusive scan, atomic slot placement
+ * and a per-cell sort of the keys give the reference order independently of
+ * the atomics' timing; a per-cell pass then accumulates vfield_new in that
+ * order (FP64 add, FP32 store, as the reference's real(4) vfield_new) and
+ * re-quantises x and v.
+ */
+#include "cube_internal.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define DT 256
+static const double WEIGHT_V = (double)0.1f; /* real(8),parameter :: weight_v=0.1 */
+
+enum { DERR_RANGE = 1, DERR_KEY = 2, DERR_BIG = 4, DERR_XOVER = 8 };
+
+/* ---------------- small utilities ---------------- */
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+static int fill_i32(cube_ctx *c, int32_t *p, int64_t n, int32_t v) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, c->stream, p, n, v);
+  CUBE_LAUNCH_CHECK();
+  return CUBE_OK;
+}
+static int fill_i64(cube_ctx *c, int64_t *p, int64_t n, int64_t v) {
+  hipLaunchKernelGGL(k_fill_i64, dim3(64), dim3(256), 0, c->stream, p, n, v);
+  CUBE_LAUNCH_CHECK();
+  return CUBE_OK;
+}
+
+/* decoded residual velocity table: vdec[code+nvbin/2] =
+   tan(pi*real(code)/real(nvbin-1)) / (sqrt(pi/2)/(sigma*vrel_boost)) */
+__global__ void k_vdec(const float *tanf_tab, double *vdec, int nv, double vc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) vdec[i] = (double)tanf_tab[i] / vc;
+}
+int cube_vdec_table(cube_ctx *c, double *dst, float sigma) {
+  const int nv = c->nvbin;
+  hipLaunchKernelGGL(k_vdec, dim3((nv + 255) / 256), dim3(256), 0, c->stream, c->tanf_tab, dst, nv,
+                     cube_vconst(sigma));
+  CUBE_LAUNCH_CHECK();
+  return CUBE_OK;
+}
+
+/* ---------------- exclusive scan of int32 counts -> int64 ---------------- */
+#define SCAN_B 1024
+__global__ __launch_bounds__(SCAN_B) void k_scan_blocks(const int32_t *cnt, int64_t n, int64_t *bsum) {
+  __shared__ int64_t s[SCAN_B];
+  const int64_t i = blockIdx.x * (int64_t)SCAN_B + threadIdx.x;
+  s[threadIdx.x] = i < n ? cnt[i] : 0;
+  __syncthreads();
+  for (int o = SCAN_B / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+__global__ __launch_bounds__(SCAN_B) void k_scan_partials(int64_t *bsum, int64_t nb, int64_t *total) {
+  /* exclusive scan of nb block sums, one block, chunked */
+  __shared__ int64_t s[SCAN_B];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < nb; base += SCAN_B) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t v = i < nb ? bsum[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_B; o <<= 1) {
+      const int64_t t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == SCAN_B - 1) carry += s[SCAN_B - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+__global__ __launch_bounds__(SCAN_B) void k_scan_final(const int32_t *cnt, int64_t n, const int64_t *boff,
+                                                     int64_t *start, const int64_t *total) {
+  __shared__ int64_t s[SCAN_B];
+  const int64_t i = blockIdx.x * (int64_t)SCAN_B + threadIdx.x;
+  const int64_t v = i < n ? cnt[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < SCAN_B; o <<= 1) {
+    const int64_t t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) start[i] = boff[blockIdx.x] + s[threadIdx.x] - v;
+  if (i == n - 1) start[n] = *total;
+}
+int cube_scan_counts(cube_ctx *c, const int32_t *cnt, int64_t *start, int64_t n, int64_t *total) {
+  const int64_t nb = (n + SCAN_B - 1) / SCAN_B;
+  int64_t *bsum = c->scanbuf;   /* nb + 1 entries */
+  int64_t *tot = c->scanbuf + nb;
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_B), 0, c->stream, cnt, n, bsum);
+  CUBE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_B), 0, c->stream, bsum, nb, tot);
+  CUBE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(SCAN_B), 0, c->stream, cnt, n, bsum, start, tot);
+  CUBE_LAUNCH_CHECK();
+  if (total) {
+    CUBE_CHECK(hipMemcpyAsync(total, tot, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  }
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  return CUBE_OK;
+}
+
+/* ---------------- per-tile sums over the extended box ---------------- */
+/* out[t] = sum of rhoc over local cells [1-w, nt+w]^3 of tile t (periodic) */
+__global__ void k_tile_boxsum(const int32_t *rhoc, Geo g, int w, unsigned long long *out) {
+  const int t = blockIdx.y;
+  const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
+  const int64_t e = g.nt + 2 * w, vol = e * e * e;
+  int64_t s = 0;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < vol; q += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % e), j = (int)((q / e) % e), k = (int)(q / (e * e));
+    s += rhoc[cid_global(g, tx * g.nt + i - w, ty * g.nt + j - w, tz * g.nt + k - w)];
+  }
+  __shared__ int64_t r[256];
+  r[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) r[threadIdx.x] += r[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out + t, (unsigned long long)r[0]);
+}
+int cube_tile_boxsum(cube_ctx *c, const int32_t *rhoc, int w, int64_t *host_out) {
+  int st = fill_i64(c, c->ired, c->ntiles, 0);
+  if (st) return st;
+  hipLaunchKernelGGL(k_tile_boxsum, dim3(256, (unsigned)c->ntiles), dim3(256), 0, c->stream, rhoc, geo_of(c), w,
+                     (unsigned long long *)c->ired);
+  CUBE_LAUNCH_CHECK();
+  CUBE_CHECK(hipMemcpyAsync(host_out, c->ired, c->ntiles * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  return CUBE_OK;
+}
+
+/* ---------------- pass 1: destination cell and order key ---------------- */
+struct DriftP {
+  Geo g;
+  int nlayer;
+  int nvh;        /* nvbin/2 */
+  float dt_mid;
+  double xres;
+};
+
+template <int IX, int IV>
+__global__ __launch_bounds__(DT) void k_drift_dest(const typename IntOf<IX>::T *__restrict__ xp,
+                                                   const typename IntOf<IV>::T *__restrict__ vp,
+                                                   const float *__restrict__ vfield,
+                                                   const int64_t *__restrict__ cstart,
+                                                   const double *__restrict__ vdec, DriftP P,
+                                                   int32_t *__restrict__ dest, uint64_t *__restrict__ key,
+                                                   int32_t *__restrict__ cnt, int32_t *__restrict__ xdest,
+                                                   uint64_t *__restrict__ xkey, int64_t xcap,
+                                                   unsigned long long *__restrict__ counters) {
+  /* counters: [0] extra copies, [1] lost, [2] error flags */
+  const Geo g = P.g;
+  const int nt = g.nt, ncb = g.ncb;
+  const int64_t row = blockIdx.x;               /* j + nt*(k + nt*t) */
+  const int j = (int)(row % nt), k = (int)((row / nt) % nt), t = (int)(row / ((int64_t)nt * nt));
+  const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
+  __shared__ int64_t cs[520];
+  const int64_t c0 = row * nt;
+  for (int i = threadIdx.x; i <= nt; i += DT) cs[i] = cstart[c0 + i];
+  __syncthreads();
+  const int64_t pbeg = cs[0], pend = cs[nt];
+  for (int64_t p = pbeg + threadIdx.x; p < pend; p += DT) {
+    /* cell of p: largest i with cs[i] <= p */
+    int lo = 0, hi = nt;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (cs[mid] <= p) lo = mid; else hi = mid;
+    }
+    const int i = lo;
+    const int home[3] = {i + 1, j + 1, k + 1};   /* 1-based tile-local */
+    const int tile[3] = {tx, ty, tz};
+    int lcoord[3][3], lig[3][3], loff[3][3], nl[3];
+    int err = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int32_t xc = (int32_t)xp[3 * p + a];
+      const int32_t vc = (int32_t)vp[3 * p + a];
+      const double off = ((double)cube_ucode(IX, xc) + 0.5) * P.xres;
+      const double vreal = vdec[vc + P.nvh] + (double)vfield[3 * (c0 + i) + a];
+      const double dx = ((double)P.dt_mid * vreal) / 4.0;
+      nl[a] = 0;
+      int cand[3] = {home[a], 0, 0}, coff[3] = {0, 0, 0}, nc = 1;
+      if (home[a] >= nt - ncb + 1) { cand[nc] = home[a] - nt; coff[nc] = 1; ++nc; }
+      if (home[a] <= ncb) { cand[nc] = home[a] + nt; coff[nc] = -1; ++nc; }
+      for (int q = 0; q < nc; ++q) {
+        const double xq = ((double)cand[q] - 1.0) + off;
+        const double ig = ceil(xq + dx);
+        if (ig < 1 - 2 * ncb || ig > nt + 2 * ncb) { err |= DERR_RANGE; continue; }
+        const int igi = (int)ig;
+        if (igi >= 1 && igi <= nt) {
+          lcoord[a][nl[a]] = cand[q];
+          lig[a][nl[a]] = igi;
+          loff[a][nl[a]] = coff[q];
+          ++nl[a];
+        }
+      }
+    }
+    const int ncopy = nl[0] * nl[1] * nl[2];
+    if (ncopy == 0) {
+      dest[p] = -1;
+      atomicAdd(counters + 1, 1ull);
+    }
+    int w = 0;
+    for (int qz = 0; qz < nl[2]; ++qz)
+      for (int qy = 0; qy < nl[1]; ++qy)
+        for (int qx = 0; qx < nl[0]; ++qx, ++w) {
+          const int q[3] = {qx, qy, qz};
+          int dt_[3], r[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            dt_[a] = wrapi(tile[a] + loff[a][q[a]], g.nnt);
+            r[a] = lcoord[a][q[a]] - lig[a][q[a]] + 2 * ncb;
+            if (r[a] < 0 || r[a] > 31) err |= DERR_KEY;
+          }
+          const int il = (lcoord[2][qz] - (1 - ncb)) % P.nlayer;
+          const int32_t dc = (int32_t)g.cid(lig[0][qx] - 1, lig[1][qy] - 1, lig[2][qz] - 1,
+                                            dt_[0] + g.nnt * (dt_[1] + g.nnt * dt_[2]));
+          const uint64_t kk = ((uint64_t)il << 55) | ((uint64_t)(r[2] & 31) << 50) |
+                              ((uint64_t)(r[1] & 31) << 45) | ((uint64_t)(r[0] & 31) << 40) | (uint64_t)p;
+          atomicAdd(cnt + dc, 1);
+          if (w == 0) {
+            dest[p] = dc;
+            key[p] = kk;
+          } else {
+            const unsigned long long x = atomicAdd(counters + 0, 1ull);
+            if ((int64_t)x < xcap) {
+              xdest[x] = dc;
+              xkey[x] = kk;
+            } else {
+              err |= DERR_XOVER;
+            }
+          }
+        }
+    if (err) atomicOr(counters + 2, (unsigned long long)err);
+  }
+}
+
+/* ---------------- placement ---------------- */
+__global__ void k_place(const int32_t *__restrict__ dest, const uint64_t *__restrict__ key, int64_t n,
+                        const int64_t *__restrict__ start, int32_t *__restrict__ fill, uint64_t *__restrict__ out) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t d = dest[p];
+    if (d < 0) continue;
+    const int32_t s = atomicAdd(fill + d, 1);
+    out[start[d] + s] = key[p];
+  }
+}
+
+/* ---------------- per-cell sort ---------------- */
+#define SMALL_SORT 48
+#define BIG_SORT 16384
+__global__ void k_sort_small(const int32_t *__restrict__ cnt, const int64_t *__restrict__ start, int64_t ncells,
+                             uint64_t *__restrict__ a, int32_t *__restrict__ biglist,
+                             unsigned long long *__restrict__ counters) {
+  const int64_t cidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (cidx >= ncells) return;
+  const int n = cnt[cidx];
+  if (n <= 1) return;
+  if (n > SMALL_SORT) {
+    const unsigned long long q = atomicAdd(counters + 3, 1ull);
+    biglist[q] = (int32_t)cidx;
+    return;
+  }
+  uint64_t v[SMALL_SORT];
+  uint64_t *s = a + start[cidx];
+  for (int q = 0; q < SMALL_SORT; ++q)
+    if (q < n) v[q] = s[q];
+  /* insertion sort in registers (n <= 48) */
+  for (int q = 1; q < SMALL_SORT; ++q) {
+    if (q >= n) break;
+    const uint64_t x = v[q];
+    int r = q - 1;
+    while (r >= 0 && v[r] > x) {
+      v[r + 1] = v[r];
+      --r;
+    }
+    v[r + 1] = x;
+  }
+  for (int q = 0; q < SMALL_SORT; ++q)
+    if (q < n) s[q] = v[q];
+}
+
+/* one workgroup per big cell: bitonic sort in LDS (n <= BIG_SORT) */
+__global__ __launch_bounds__(1024) void k_sort_big(const int32_t *__restrict__ cnt,
+                                                   const int64_t *__restrict__ start,
+                                                   const int32_t *__restrict__ biglist, uint64_t *__restrict__ a,
+                                                   unsigned long long *__restrict__ counters) {
+  extern __shared__ uint64_t sk[];
+  const int32_t cidx = biglist[blockIdx.x];
+  const int n = cnt[cidx];
+  if (n > BIG_SORT) {
+    if (threadIdx.x == 0) atomicOr(counters + 2, (unsigned long long)DERR_BIG);
+    return;
+  }
+  int m = 1;
+  while (m < n) m <<= 1;
+  uint64_t *s = a + start[cidx];
+  for (int q = threadIdx.x; q < m; q += blockDim.x) sk[q] = q < n ? s[q] : ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= m; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int q = threadIdx.x; q < m; q += blockDim.x) {
+        const int o = q ^ stride;
+        if (o > q) {
+          const bool up = (q & size) == 0;
+          const uint64_t x = sk[q], y = sk[o];
+          if ((x > y) == up) {
+            sk[q] = y;
+            sk[o] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int q = threadIdx.x; q < n; q += blockDim.x) s[q] = sk[q];
+}
+
+/* ---------------- per-cell finalize ---------------- */
+template <int IX, int IV>
+__global__ void k_drift_final(const typename IntOf<IX>::T *__restrict__ xp, const typename IntOf<IV>::T *__restrict__ vp,
+                              const int32_t *__restrict__ pid, const float *__restrict__ vfield_old,
+                              const int32_t *__restrict__ cnt, const int64_t *__restrict__ start,
+                              const uint64_t *__restrict__ keys, const double *__restrict__ vdec, DriftP P, double venc,
+                              typename IntOf<IX>::T *__restrict__ xp2, typename IntOf<IV>::T *__restrict__ vp2,
+                              int32_t *__restrict__ pid2, float *__restrict__ vfield_new, int64_t ncells) {
+  const int64_t cidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (cidx >= ncells) return;
+  const Geo g = P.g;
+  const int nt = g.nt, ncb = g.ncb;
+  const int i = (int)(cidx % nt), j = (int)((cidx / nt) % nt), k = (int)((cidx / ((int64_t)nt * nt)) % nt);
+  const int t = (int)(cidx / ((int64_t)nt * nt * nt));
+  const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
+  const int n = cnt[cidx];
+  const int64_t s0 = start[cidx];
+  float acc[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) acc[a] = (float)((double)vfield_old[3 * cidx + a] * WEIGHT_V);
+  for (int q = 0; q < n; ++q) {
+    const uint64_t kk = keys[s0 + q];
+    const int64_t p = (int64_t)(kk & KEY_PMASK);
+    const int rx = (int)((kk >> 40) & 31), ry = (int)((kk >> 45) & 31), rz = (int)((kk >> 50) & 31);
+    /* source cell = destination + (r - 2ncb), in the destination tile frame */
+    const int64_t sc = cid_global(g, tx * nt + i + rx - 2 * ncb, ty * nt + j + ry - 2 * ncb,
+                                  tz * nt + k + rz - 2 * ncb);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double vreal = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vfield_old[3 * sc + a];
+      acc[a] = (float)((double)acc[a] + vreal);
+    }
+  }
+  float vn[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    vn[a] = (float)((double)acc[a] / ((double)n + WEIGHT_V));
+    vfield_new[3 * cidx + a] = vn[a];
+  }
+  for (int q = 0; q < n; ++q) {
+    const uint64_t kk = keys[s0 + q];
+    const int64_t p = (int64_t)(kk & KEY_PMASK);
+    const int rx = (int)((kk >> 40) & 31), ry = (int)((kk >> 45) & 31), rz = (int)((kk >> 50) & 31);
+    const int64_t sc = cid_global(g, tx * nt + i + rx - 2 * ncb, ty * nt + j + ry - 2 * ncb,
+                                  tz * nt + k + rz - 2 * ncb);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double vreal = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vfield_old[3 * sc + a];
+      /* xp_new=xp+nint(dt_mid*vreal/(x_resolution*ncell)) (:115) */
+      const double qx = ((double)P.dt_mid * vreal) / (P.xres * 4.0);
+      xp2[3 * (s0 + q) + a] = (typename IntOf<IX>::T)cube_wrap(IX, (int64_t)xp[3 * p + a] + (int64_t)cube_nint(qx));
+      /* vp_new=nint(...atan(...*(vreal-vfield_new))...) (:116-117) */
+      vp2[3 * (s0 + q) + a] = (typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal - (double)vn[a]);
+    }
+    if (pid) pid2[s0 + q] = pid[p];
+  }
+}
+
+/* ---------------- velocity statistics (:172-224) ---------------- */
+#define VS_BLOCKS 1024
+template <int IV>
+__global__ __launch_bounds__(256) void k_vstats(const typename IntOf<IV>::T *__restrict__ vp,
+                                                const float *__restrict__ vfield, const int32_t *__restrict__ cnt,
+                                                const int64_t *__restrict__ start, const double *__restrict__ vdec,
+                                                int nvh, int64_t ncells, double *__restrict__ part) {
+  double sc = 0, sr = 0, ss = 0;
+  for (int64_t cidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cidx < ncells;
+       cidx += (int64_t)gridDim.x * blockDim.x) {
+    const float v0 = vfield[3 * cidx], v1 = vfield[3 * cidx + 1], v2 = vfield[3 * cidx + 2];
+    sc += (double)((v0 * v0 + v1 * v1) + v2 * v2);
+    const int n = cnt[cidx];
+    const int64_t s0 = start[cidx];
+    for (int q = 0; q < n; ++q) {
+      double r[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) r[a] = vdec[(int32_t)vp[3 * (s0 + q) + a] + nvh];
+      sr += (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+      r[0] += (double)v0;
+      r[1] += (double)v1;
+      r[2] += (double)v2;
+      ss += (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+    }
+  }
+  __shared__ double red[3][256];
+  red[0][threadIdx.x] = sc;
+  red[1][threadIdx.x] = sr;
+  red[2][threadIdx.x] = ss;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ void k_sum_parts(const double *part, int nb, double *out) {
+  /* fixed-order final sum of the block partials */
+  if (threadIdx.x < 3) {
+    double s = 0;
+    for (int b = 0; b < nb; ++b) s += part[3 * b + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+int cube_vstats_run(cube_ctx *c, float sigma_vi, double *sums3) {
+  int st = cube_vdec_table(c, c->vdec0, sigma_vi);
+  if (st) return st;
+  const int nvh = c->nvbin / 2;
+  if (c->p.izipv == 1)
+    hipLaunchKernelGGL(k_vstats<1>, dim3(VS_BLOCKS), dim3(256), 0, c->stream, (const int8_t *)c->vp, c->vfield,
+                       c->rhoc, c->cstart, c->vdec0, nvh, c->ncells, c->dred);
+  else
+    hipLaunchKernelGGL(k_vstats<2>, dim3(VS_BLOCKS), dim3(256), 0, c->stream, (const int16_t *)c->vp, c->vfield,
+                       c->rhoc, c->cstart, c->vdec0, nvh, c->ncells, c->dred);
+  CUBE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, c->stream, c->dred, VS_BLOCKS, c->dred + 3 * VS_BLOCKS);
+  CUBE_LAUNCH_CHECK();
+  CUBE_CHECK(hipMemcpyAsync(sums3, c->dred + 3 * VS_BLOCKS, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  return CUBE_OK;
+}
+
+/* ---------------- driver ---------------- */
+template <int IX, int IV> static int drift_impl(cube_ctx *c, DriftP P, double venc, int64_t *nxtra,
+                                                unsigned long long *hc) {
+  typedef typename IntOf<IX>::T TX;
+  typedef typename IntOf<IV>::T TV;
+  unsigned long long *ctr = (unsigned long long *)c->ured;
+  int st;
+  if ((st = fill_i32(c, c->rhoc2, c->ncells, 0))) return st;
+  if ((st = fill_i32(c, c->fill, c->ncells, 0))) return st;
+  if ((st = fill_i64(c, (int64_t *)ctr, 8, 0))) return st;
+  const int64_t nrows = c->ncells / c->nt;
+  hipLaunchKernelGGL((k_drift_dest<IX, IV>), dim3((unsigned)nrows), dim3(DT), 0, c->stream, (const TX *)c->xp,
+                     (const TV *)c->vp, c->vfield, c->cstart, c->vdec0, P, c->dest, c->key, c->rhoc2, c->xdest,
+                     c->xkey, c->xcap, ctr);
+  CUBE_LAUNCH_CHECK();
+  int64_t total = 0;
+  if ((st = cube_scan_counts(c, c->rhoc2, c->cstart2, c->ncells, &total))) return st;
+  CUBE_CHECK(hipMemcpy(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  if (hc[2] & (DERR_RANGE | DERR_KEY)) return CUBE_ERR_DISPLACEMENT;
+  if (hc[2] & DERR_XOVER) return CUBE_ERR_IMAGE_OVERFLOW;
+  if (total > c->cap) return CUBE_ERR_IMAGE_OVERFLOW;
+  *nxtra = (int64_t)hc[0];
+  hipLaunchKernelGGL(k_place, dim3(4096), dim3(256), 0, c->stream, c->dest, c->key, c->nplocal, c->cstart2, c->fill,
+                     c->sortbuf);
+  CUBE_LAUNCH_CHECK();
+  if (hc[0]) {
+    hipLaunchKernelGGL(k_place, dim3(64), dim3(256), 0, c->stream, c->xdest, c->xkey, (int64_t)hc[0], c->cstart2,
+                       c->fill, c->sortbuf);
+    CUBE_LAUNCH_CHECK();
+  }
+  const unsigned nbc = (unsigned)((c->ncells + 255) / 256);
+  int32_t *biglist = (int32_t *)c->dest; /* dest is dead after placement */
+  hipLaunchKernelGGL(k_sort_small, dim3(nbc), dim3(256), 0, c->stream, c->rhoc2, c->cstart2, c->ncells, c->sortbuf,
+                     biglist, ctr);
+  CUBE_LAUNCH_CHECK();
+  CUBE_CHECK(hipMemcpy(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  if (hc[3]) {
+    hipLaunchKernelGGL(k_sort_big, dim3((unsigned)hc[3]), dim3(1024), BIG_SORT * sizeof(uint64_t), c->stream,
+                       c->rhoc2, c->cstart2, biglist, c->sortbuf, ctr);
+    CUBE_LAUNCH_CHECK();
+    CUBE_CHECK(hipMemcpy(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (hc[2] & DERR_BIG) return CUBE_ERR_INVALID;
+  }
+  hipLaunchKernelGGL((k_drift_final<IX, IV>), dim3(nbc), dim3(256), 0, c->stream, (const TX *)c->xp, (const TV *)c->vp,
+                     c->p.has_pid ? c->pid : nullptr, c->vfield, c->rhoc2, c->cstart2, c->sortbuf, c->vdec0, P, venc,
+                     (TX *)c->xp2, (TV *)c->vp2, c->pid2, c->vfield2, c->ncells);
+  CUBE_LAUNCH_CHECK();
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  c->nplocal = total;
+  return CUBE_OK;
+}
+
+int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi, cube_vstats *out) {
+  /* dt_mid=(dt_old+dt)/2; nlayer=2*ceiling(dt_mid*vz_max/ncell)+1 (real(4)) */
+  const float dt_mid = (dt_old + dt) / 2.0f;
+  const int nlayer = 2 * (int)ceilf(dt_mid * vz_max / 4.0f) + 1;
+  if (nlayer < 1 || nlayer > 511) return CUBE_ERR_DISPLACEMENT;
+  /* tile overflow check (W/update_particle.f90:86-94): the copies in each
+     tile's extended region must fit np_tile_max */
+  int64_t *sums = (int64_t *)malloc(c->ntiles * sizeof(int64_t));
+  int st = cube_tile_boxsum(c, c->rhoc, (int)c->ncb, sums);
+  double ov = 0;
+  int64_t mx = 0;
+  for (int64_t t = 0; t < c->ntiles; ++t) {
+    const double o = (double)((float)sums[t] / (float)c->np_tile_max);
+    ov = o > ov ? o : ov;
+    mx = sums[t] > mx ? sums[t] : mx;
+  }
+  free(sums);
+  if (st) return st;
+  if (mx > c->np_tile_max) return CUBE_ERR_TILE_OVERFLOW;
+  if ((st = cube_vdec_table(c, c->vdec0, sigma_vi))) return st;
+  DriftP P;
+  P.g = geo_of(c);
+  P.nlayer = nlayer;
+  P.nvh = c->nvbin / 2;
+  P.dt_mid = dt_mid;
+  P.xres = c->p.izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0;
+  const double venc = cube_vconst(sigma_vi);
+  int64_t nxtra = 0;
+  unsigned long long hc[8];
+  const int64_t np_old = c->nplocal;
+  if (c->p.izipx == 1 && c->p.izipv == 1) st = drift_impl<1, 1>(c, P, venc, &nxtra, hc);
+  else if (c->p.izipx == 1) st = drift_impl<1, 2>(c, P, venc, &nxtra, hc);
+  else if (c->p.izipv == 1) st = drift_impl<2, 1>(c, P, venc, &nxtra, hc);
+  else st = drift_impl<2, 2>(c, P, venc, &nxtra, hc);
+  if (st) return st;
+  /* the new state becomes current */
+  void *tx = c->xp; c->xp = c->xp2; c->xp2 = tx;
+  void *tv = c->vp; c->vp = c->vp2; c->vp2 = tv;
+  int32_t *tp = c->pid; c->pid = c->pid2; c->pid2 = tp;
+  int32_t *tr = c->rhoc; c->rhoc = c->rhoc2; c->rhoc2 = tr;
+  float *tf = c->vfield; c->vfield = c->vfield2; c->vfield2 = tf;
+  int64_t *ts = c->cstart; c->cstart = c->cstart2; c->cstart2 = ts;
+  /* std of the velocity field, FP64 sums (:171-224) */
+  double s3[3];
+  if ((st = cube_vstats_run(c, sigma_vi, s3))) return st;
+  const double npg = (double)c->npglobal;
+  cube_vstats v;
+  memset(&v, 0, sizeof(v));
+  v.sum_c = s3[0];
+  v.sum_r = s3[1];
+  v.sum_s = s3[2];
+  v.std_vsim = sqrt(s3[2] / npg);
+  v.std_vsim_c = sqrt(s3[0] / (double)c->nc / (double)c->nc / (double)c->nc);
+  v.std_vsim_res = sqrt(s3[1] / npg);
+  v.sigma_vi_new = (float)(v.std_vsim_res / (double)sqrtf(3.0f));
+  v.nlayer = nlayer;
+  v.nplocal = c->nplocal;
+  v.lost = (int64_t)hc[1];
+  v.overhead_tile = ov;
+  (void)np_old;
+  if (out) *out = v;
+  if (c->nplocal != c->npglobal) return CUBE_ERR_COUNT;
+  return CUBE_OK;
+}

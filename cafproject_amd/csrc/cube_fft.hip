@@ -1,0 +1,580 @@
+/*
+ * cube_fft.hip -- 3D real FFTs of the periodic meshes with the Green's
+ * function multiply fused in, replacing the reference's cube_fft
+ * (W/cubefft.f90:15-25, FFTW r2c/c2r of the buffered fine tile) and
+ * pencil_fft (W/pencil_fft.f90:31-60) for a single image.
+ *
+ * Conventions match FFTW's as the reference uses them: forward unnormalised
+ * (e^{-i}), backward unnormalised (e^{+i}) followed by three divisions by N
+ * in real(4) (W/pm.f90:123, W/pencil_fft.f90:58).
+ *
+ * Structure (mesh N^3, x fastest; half spectrum NH=N/2+1 along x):
+ *   k_r2c_x   : rows along x, real N -> complex N/2 FFT + split -> NH
+ *   k_c2c_s   : lines along y (stride NH) or z (stride N*NH), in place
+ *   k_green_z : lines along z: forward FFT, then for d=0..2 multiply by
+ *               i*K_d(k) and inverse FFT, writing three spectra
+ *   k_c2r_x3  : rows along x for the three spectra: merge -> complex N/2
+ *               inverse FFT -> real, /N/N/N, write F as 3 floats per cell,
+ *               and max |F|^2 (W/pm.f90:126,315)
+ * Every line FFT is a Stockham autosort FFT in LDS (radix 8/4/2/3/5/7),
+ * a batch of B lines per 256-thread workgroup; strided passes load B
+ * consecutive x-columns per row so global accesses stay coalesced.
+ */
+#include "cube_internal.h"
+#include <string.h>
+#include <math.h>
+
+#define FFT_T 256
+
+struct FFTPlan {
+  int n;
+  float2 *tw; /* n twiddles exp(-2 pi i m/n) */
+};
+
+/* ---------------- compile-time plans ---------------- */
+__host__ __device__ constexpr int pick_radix(int rem) {
+  return (rem % 8 == 0 && rem != 16) ? 8
+         : (rem % 4 == 0)            ? 4
+         : (rem % 2 == 0)            ? 2
+         : (rem % 3 == 0)            ? 3
+         : (rem % 5 == 0)            ? 5
+         : (rem % 7 == 0)            ? 7
+                                     : rem;
+}
+__host__ __device__ constexpr int radix_at(int L, int i) {
+  int rem = L;
+  for (int k = 0; k < i; ++k) rem /= pick_radix(rem);
+  return rem > 1 ? pick_radix(rem) : 1;
+}
+__host__ __device__ constexpr int ns_at(int L, int i) { /* product of radices before pass i */
+  int ns = 1, rem = L;
+  for (int k = 0; k < i; ++k) {
+    const int r = pick_radix(rem);
+    ns *= r;
+    rem /= r;
+  }
+  return ns;
+}
+__host__ __device__ constexpr int npass(int L) {
+  int n = 0, rem = L;
+  while (rem > 1) {
+    rem /= pick_radix(rem);
+    ++n;
+  }
+  return n;
+}
+
+/* ---------------- complex helpers ---------------- */
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+/* multiply by -i (forward, DIR=-1) or +i (inverse, DIR=+1) */
+template <int DIR> __device__ __forceinline__ float2 rot(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int R, int DIR> struct DFT;
+template <int DIR> struct DFT<2, DIR> {
+  __device__ __forceinline__ static void run(float2 *v) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+template <int DIR> struct DFT<4, DIR> {
+  __device__ __forceinline__ static void run(float2 *v) {
+    const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const float2 t2 = cadd(v[1], v[3]), t3 = rot<DIR>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+template <int DIR> struct DFT<8, DIR> {
+  __device__ __forceinline__ static void run(float2 *v) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    DFT<4, DIR>::run(e);
+    DFT<4, DIR>::run(o);
+    const float h = 0.70710678118654752f;
+    /* W8^k = exp(DIR*2 pi i k/8) */
+    const float2 w1 = make_float2(h, DIR * h), w3 = make_float2(-h, DIR * h);
+    o[1] = cmul(o[1], w1);
+    o[2] = rot<DIR>(o[2]);
+    o[3] = cmul(o[3], w3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+template <int DIR> struct DFT<3, DIR> {
+  __device__ __forceinline__ static void run(float2 *v) {
+    const float s = 0.86602540378443865f;
+    const float2 a = v[0], b = v[1], c = v[2];
+    const float2 sum = cadd(b, c);
+    const float2 t = make_float2(a.x - 0.5f * sum.x, a.y - 0.5f * sum.y);
+    const float2 d = csub(b, c);
+    const float2 u = rot<DIR>(make_float2(s * d.x, s * d.y));
+    v[0] = cadd(a, sum);
+    v[1] = cadd(t, u);
+    v[2] = csub(t, u);
+  }
+};
+/* generic odd prime radix by direct summation */
+template <int R, int DIR> struct DFTG {
+  __device__ __forceinline__ static void run(float2 *v) {
+    float2 out[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        const int m = (k * q) % R;
+        const float ang = DIR * 2.0f * 3.14159265358979f * (float)m / (float)R;
+        acc = cadd(acc, cmul(v[q], make_float2(cosf(ang), sinf(ang))));
+      }
+      out[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+  }
+};
+template <int DIR> struct DFT<5, DIR> : DFTG<5, DIR> {};
+template <int DIR> struct DFT<7, DIR> : DFTG<7, DIR> {};
+
+/*
+ * One Stockham pass over a line of length L held in LDS at `buf` (unit
+ * stride).  tl = thread index within the line, TL threads per line.
+ * Pass with sub-transform size NS: butterfly j takes buf[j + q*L/R],
+ * twiddles by exp(DIR 2 pi i k q/(NS R)), k=j%NS, and stores to
+ * buf[(j/NS)*NS*R + k + q*NS].  Values are staged in registers so the pass
+ * is in place; the caller's barrier pattern is inside.
+ */
+template <int L, int R, int NS, int DIR, int TL>
+__device__ __forceinline__ void stockham_pass(float2 *buf, int tl, const float2 *__restrict__ tw,
+                                              int nt_tab) {
+  constexpr int NB = L / R;
+  constexpr int PER = (NB + TL - 1) / TL;
+  float2 v[PER][R];
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int j = tl + b * TL;
+    if (PER * TL == NB || j < NB) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[b][q] = buf[j + q * NB];
+    }
+  }
+  __syncthreads();
+  const int twmul = nt_tab / (NS * R);
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int j = tl + b * TL;
+    if (PER * TL == NB || j < NB) {
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          float2 w = tw[k * q * twmul];
+          if (DIR > 0) w.y = -w.y;
+          v[b][q] = cmul(v[b][q], w);
+        }
+      }
+      DFT<R, DIR>::run(v[b]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[base + q * NS] = v[b][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, int DIR, int TL, int P>
+__device__ __forceinline__ void fft_passes(float2 *buf, int tl, const float2 *tw, int nt_tab) {
+  if constexpr (P < npass(L)) {
+    stockham_pass<L, radix_at(L, P), ns_at(L, P), DIR, TL>(buf, tl, tw, nt_tab);
+    fft_passes<L, DIR, TL, P + 1>(buf, tl, tw, nt_tab);
+  }
+}
+/* full line FFT (natural order in and out) */
+template <int L, int DIR, int TL>
+__device__ __forceinline__ void line_fft(float2 *buf, int tl, const float2 *tw, int nt_tab) {
+  fft_passes<L, DIR, TL, 0>(buf, tl, tw, nt_tab);
+}
+
+/* ---------------- batch sizes ---------------- */
+template <int N> struct Cfg {
+  static constexpr int L = N / 2;
+  /* strided passes: B lines of length N */
+  static constexpr int BS = N <= 512 ? 16 : (N <= 1024 ? 8 : 4);
+  /* contiguous passes: B rows of complex length L */
+  static constexpr int BC0 = 2048 / L;
+  static constexpr int BC = BC0 < 1 ? 1 : (BC0 > 16 ? 16 : BC0);
+};
+
+/* ---------------- kernels ---------------- */
+/* real rows (N floats) -> half spectra rows (NH complex) */
+template <int N>
+__global__ __launch_bounds__(FFT_T) void k_r2c_x(const float *__restrict__ in, float2 *__restrict__ out,
+                                               const float2 *__restrict__ tw, int64_t nrows) {
+  constexpr int L = N / 2, B = Cfg<N>::BC, TL = FFT_T / B, LP = L + 1;
+  __shared__ float2 sm[B * LP];
+  const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
+  const int64_t row0 = (int64_t)blockIdx.x * B;
+  const float2 *in2 = reinterpret_cast<const float2 *>(in);
+  for (int idx = tid; idx < B * L; idx += FFT_T) {
+    const int l = idx / L, m = idx % L;
+    sm[l * LP + m] = (row0 + l < nrows) ? in2[(row0 + l) * L + m] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  line_fft<L, -1, TL>(sm + line * LP, tl, tw, N);
+  /* split: X[k] = (Z[k]+conj(Z[L-k]))/2 + W^k (Z[k]-conj(Z[L-k]))/(2i) */
+  for (int idx = tid; idx < B * (L + 1); idx += FFT_T) {
+    const int l = idx / (L + 1), k = idx % (L + 1);
+    const float2 zk = sm[l * LP + (k == L ? 0 : k)];
+    const float2 zc = sm[l * LP + (k == 0 ? 0 : L - k)];
+    const float2 e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y - zc.y));
+    const float2 o = make_float2(0.5f * (zk.y + zc.y), -0.5f * (zk.x - zc.x));
+    const float2 w = tw[k];
+    const float2 wo = cmul(w, o);
+    if (row0 + l < nrows) out[(row0 + l) * (L + 1) + k] = cadd(e, wo);
+  }
+}
+
+/* in-place strided line FFTs: element (col, n) at base + n*stride + col,
+   col in [0, ncol); grid.x = column blocks, grid.y = outer index with
+   outer stride `ostride` */
+template <int N, int DIR>
+__global__ __launch_bounds__(FFT_T) void k_c2c_s(float2 *__restrict__ a, const float2 *__restrict__ tw,
+                                               int ncol, int64_t stride, int64_t ostride) {
+  constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = N + 1;
+  __shared__ float2 sm[B * NP];
+  const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
+  const int col0 = blockIdx.x * B;
+  float2 *base = a + (int64_t)blockIdx.y * ostride + col0;
+  for (int idx = tid; idx < B * N; idx += FFT_T) {
+    const int l = idx % B, n = idx / B;
+    sm[l * NP + n] = (col0 + l < ncol) ? base[n * stride + l] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  line_fft<N, DIR, TL>(sm + line * NP, tl, tw, N);
+  for (int idx = tid; idx < B * N; idx += FFT_T) {
+    const int l = idx % B, n = idx / B;
+    if (col0 + l < ncol) base[n * stride + l] = sm[l * NP + n];
+  }
+}
+
+/* Green's function pass along z.  spec: [z][y][x'] (x' < NH).  For each
+   (x',y) line: forward FFT along z, then for d: out_d = IFFT_z(i K_d * .).
+   kern: octant storage [d][kz'][ky'][kx] of (N/2+1)^3 (octant=1, odd in the
+   d-th wave number) or full storage [d][kz][ky][kx] of N*N*NH (octant=0). */
+template <int N>
+__global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ spec, float2 *__restrict__ outd,
+                                                 const float *__restrict__ kern, int octant,
+                                                 const float2 *__restrict__ tw) {
+  constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = N + 1, NH = N / 2 + 1, H = N / 2;
+  constexpr int EPT = (B * N + FFT_T - 1) / FFT_T;
+  __shared__ float2 sm[B * NP];
+  const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
+  const int col0 = blockIdx.x * B, y = blockIdx.y;
+  const int64_t stride = (int64_t)N * NH;   /* between z planes */
+  const int64_t plane = (int64_t)NH * N * N;
+  const float2 *src = spec + (int64_t)y * NH + col0;
+  for (int idx = tid; idx < B * N; idx += FFT_T) {
+    const int l = idx % B, n = idx / B;
+    sm[l * NP + n] = (col0 + l < NH) ? src[n * stride + l] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  line_fft<N, -1, TL>(sm + line * NP, tl, tw, N);
+  float2 keep[EPT];
+#pragma unroll
+  for (int s = 0; s < EPT; ++s) {
+    const int idx = tid + s * FFT_T;
+    if (idx < B * N) keep[s] = sm[(idx % B) * NP + idx / B];
+  }
+  const int ky = y <= H ? y : N - y;
+  const float sy = y <= H ? 1.f : -1.f;
+  for (int d = 0; d < 3; ++d) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < EPT; ++s) {
+      const int idx = tid + s * FFT_T;
+      if (idx < B * N) {
+        const int l = idx % B, n = idx / B, kx = col0 + l;
+        float K = 0.f;
+        if (kx < NH) {
+          if (octant) {
+            const int kz = n <= H ? n : N - n;
+            const float sz = n <= H ? 1.f : -1.f;
+            K = kern[(((int64_t)d * (H + 1) + kz) * (H + 1) + ky) * (H + 1) + kx];
+            if (d == 1) K *= sy;
+            if (d == 2) K *= sz;
+          } else {
+            K = kern[(((int64_t)d * N + n) * N + y) * NH + kx];
+          }
+        }
+        const float2 r = keep[s];
+        /* rho_f(::2)=-crho(2::2)*K, rho_f(2::2)=crho(::2)*K  (W/pm.f90:119-120) */
+        sm[l * NP + n] = make_float2((-r.y) * K, r.x * K);
+      }
+    }
+    __syncthreads();
+    line_fft<N, +1, TL>(sm + line * NP, tl, tw, N);
+    float2 *dst = outd + d * plane + (int64_t)y * NH + col0;
+    for (int idx = tid; idx < B * N; idx += FFT_T) {
+      const int l = idx % B, n = idx / B;
+      if (col0 + l < NH) dst[n * stride + l] = sm[l * NP + n];
+    }
+  }
+}
+
+/* half spectra rows -> real rows for the three components, scaled by
+   three divisions by `div`; F written as 3 floats per cell; f2max gets the
+   max of (Fx^2+Fy^2)+Fz^2 as float bits. */
+template <int N>
+__global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ specd, float *__restrict__ force,
+                                                const float2 *__restrict__ tw, int64_t nrows, float div,
+                                                unsigned int *__restrict__ f2max) {
+  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = L + 1;
+  __shared__ float2 sm[3 * B * LP];
+  __shared__ float red[FFT_T];
+  const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
+  const int64_t row0 = (int64_t)blockIdx.x * B;
+  const int64_t plane = nrows * (L + 1);
+  for (int d = 0; d < 3; ++d) {
+    const float2 *in = specd + d * plane;
+    /* Z[k] = E + i O, E = X[k]+conj(X[L-k]), O = (X[k]-conj(X[L-k])) W^-k */
+    for (int idx = tid; idx < B * L; idx += FFT_T) {
+      const int l = idx / L, k = idx % L;
+      float2 z = make_float2(0.f, 0.f);
+      if (row0 + l < nrows) {
+        const float2 xk = in[(row0 + l) * (L + 1) + k];
+        const float2 xc = in[(row0 + l) * (L + 1) + (L - k)];
+        const float2 e = make_float2(xk.x + xc.x, xk.y - xc.y);
+        float2 w = tw[k];
+        w.y = -w.y;
+        const float2 o = cmul(make_float2(xk.x - xc.x, xk.y + xc.y), w);
+        z = make_float2(e.x - o.y, e.y + o.x);
+      }
+      sm[(d * B + l) * LP + k] = z;
+    }
+  }
+  __syncthreads();
+  for (int d = 0; d < 3; ++d) line_fft<L, +1, TL>(sm + (d * B + line) * LP, tl, tw, N);
+  float m = 0.f;
+  for (int idx = tid; idx < B * N; idx += FFT_T) {
+    const int l = idx / N, x = idx % N;
+    float f[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float2 z = sm[(d * B + l) * LP + (x >> 1)];
+      float v = (x & 1) ? z.y : z.x;
+      v = v / div;
+      v = v / div;
+      v = v / div;
+      f[d] = v;
+    }
+    if (row0 + l < nrows) {
+      float *o = force + ((row0 + l) * N + x) * 3;
+      o[0] = f[0];
+      o[1] = f[1];
+      o[2] = f[2];
+      const float f2 = (f[0] * f[0] + f[1] * f[1]) + f[2] * f[2];
+      m = fmaxf(m, f2);
+    }
+  }
+  red[tid] = m;
+  __syncthreads();
+  for (int s = FFT_T / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  if (tid == 0) atomicMax(f2max, __float_as_uint(red[0]));
+}
+
+/* single-component c2r (no Green's function), for inverse_full */
+template <int N>
+__global__ __launch_bounds__(FFT_T) void k_c2r_x1(const float2 *__restrict__ spec, float *__restrict__ out,
+                                                const float2 *__restrict__ tw, int64_t nrows) {
+  constexpr int L = N / 2, B = Cfg<N>::BC, TL = FFT_T / B, LP = L + 1;
+  __shared__ float2 sm[B * LP];
+  const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
+  const int64_t row0 = (int64_t)blockIdx.x * B;
+  for (int idx = tid; idx < B * L; idx += FFT_T) {
+    const int l = idx / L, k = idx % L;
+    float2 z = make_float2(0.f, 0.f);
+    if (row0 + l < nrows) {
+      const float2 xk = spec[(row0 + l) * (L + 1) + k];
+      const float2 xc = spec[(row0 + l) * (L + 1) + (L - k)];
+      const float2 e = make_float2(xk.x + xc.x, xk.y - xc.y);
+      float2 w = tw[k];
+      w.y = -w.y;
+      const float2 o = cmul(make_float2(xk.x - xc.x, xk.y + xc.y), w);
+      z = make_float2(e.x - o.y, e.y + o.x);
+    }
+    sm[l * LP + k] = z;
+  }
+  __syncthreads();
+  line_fft<L, +1, TL>(sm + line * LP, tl, tw, N);
+  for (int idx = tid; idx < B * N; idx += FFT_T) {
+    const int l = idx / N, x = idx % N;
+    const float2 z = sm[l * LP + (x >> 1)];
+    if (row0 + l < nrows) out[(row0 + l) * N + x] = (x & 1) ? z.y : z.x;
+  }
+}
+
+__global__ void k_zero_u32(unsigned *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+/* ---------------- host side ---------------- */
+int cube_fft_create(cube_ctx *c, int n, FFTPlan **out) {
+  (void)c;
+  FFTPlan *p = new FFTPlan;
+  p->n = n;
+  float2 *h = new float2[n];
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * 3.14159265358979323846 * (double)m / (double)n;
+    h[m] = make_float2((float)cos(a), (float)sin(a));
+  }
+  if (hipMalloc(&p->tw, n * sizeof(float2)) != hipSuccess) {
+    delete[] h;
+    delete p;
+    return CUBE_ERR_ALLOC;
+  }
+  CUBE_CHECK(hipMemcpy(p->tw, h, n * sizeof(float2), hipMemcpyHostToDevice));
+  delete[] h;
+  *out = p;
+  return CUBE_OK;
+}
+
+void cube_fft_destroy(FFTPlan *p) {
+  if (!p) return;
+  (void)hipFree(p->tw);
+  delete p;
+}
+
+template <int N> struct Launch {
+  static int r2c(cube_ctx *c, FFTPlan *p, const float *in, float2 *out) {
+    const int64_t rows = (int64_t)N * N;
+    const int64_t nb = (rows + Cfg<N>::BC - 1) / Cfg<N>::BC;
+    hipLaunchKernelGGL(k_r2c_x<N>, dim3((unsigned)nb), dim3(FFT_T), 0, c->stream, in, out, p->tw, rows);
+    CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
+  template <int DIR> static int c2c_y(cube_ctx *c, FFTPlan *p, float2 *a) {
+    constexpr int NH = N / 2 + 1;
+    dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
+    hipLaunchKernelGGL((k_c2c_s<N, DIR>), g, dim3(FFT_T), 0, c->stream, a, p->tw, NH, (int64_t)NH,
+                       (int64_t)NH * N);
+    CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
+  template <int DIR> static int c2c_z(cube_ctx *c, FFTPlan *p, float2 *a) {
+    constexpr int NH = N / 2 + 1;
+    dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
+    hipLaunchKernelGGL((k_c2c_s<N, DIR>), g, dim3(FFT_T), 0, c->stream, a, p->tw, NH,
+                       (int64_t)NH * N, (int64_t)NH);
+    CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
+  static int green(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int oct, float2 *work3) {
+    constexpr int NH = N / 2 + 1;
+    dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
+    hipLaunchKernelGGL(k_green_z<N>, g, dim3(FFT_T), 0, c->stream, spec, work3, kern, oct, p->tw);
+    CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
+  static int c2r3(cube_ctx *c, FFTPlan *p, const float2 *work3, float *force, float div, unsigned *f2) {
+    constexpr int BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC;
+    const int64_t rows = (int64_t)N * N;
+    hipLaunchKernelGGL(k_c2r_x3<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0, c->stream, work3,
+                       force, p->tw, rows, div, f2);
+    CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
+  static int c2r1(cube_ctx *c, FFTPlan *p, const float2 *spec, float *out) {
+    const int64_t rows = (int64_t)N * N;
+    hipLaunchKernelGGL(k_c2r_x1<N>, dim3((unsigned)((rows + Cfg<N>::BC - 1) / Cfg<N>::BC)), dim3(FFT_T), 0,
+                       c->stream, spec, out, p->tw, rows);
+    CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
+};
+
+#define FFT_SIZES(X) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512) X(1024) X(2048)
+
+int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out) {
+  int st = CUBE_ERR_INVALID;
+  switch (p->n) {
+#define CASE(N)                                                           \
+  case N:                                                                 \
+    if ((st = Launch<N>::r2c(c, p, in, out))) return st;                  \
+    return Launch<N>::c2c_y<-1>(c, p, out);
+    FFT_SIZES(CASE)
+#undef CASE
+  }
+  return st;
+}
+
+int cube_fft_forward_full(cube_ctx *c, FFTPlan *p, const float *in, float2 *out) {
+  int st = cube_fft_r2c_3d(c, p, in, out);
+  if (st) return st;
+  switch (p->n) {
+#define CASE(N) \
+  case N:       \
+    return Launch<N>::c2c_z<-1>(c, p, out);
+    FFT_SIZES(CASE)
+#undef CASE
+  }
+  return CUBE_ERR_INVALID;
+}
+
+int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out) {
+  switch (p->n) {
+#define CASE(N)                                                        \
+  case N: {                                                            \
+    int st;                                                            \
+    if ((st = Launch<N>::c2c_z<+1>(c, p, spec))) return st;            \
+    if ((st = Launch<N>::c2c_y<+1>(c, p, spec))) return st;            \
+    return Launch<N>::c2r1(c, p, spec, out);                           \
+  }
+    FFT_SIZES(CASE)
+#undef CASE
+  }
+  return CUBE_ERR_INVALID;
+}
+
+/* spec: output of cube_fft_r2c_3d (x and y transformed).  Produces the
+   three force components (3 floats per cell) and max |F|^2. */
+int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
+                          float2 *work3, float *force3, float div, float *f2max) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, (unsigned *)c->ured, 1);
+  CUBE_LAUNCH_CHECK();
+  const int64_t nh = p->n / 2 + 1, plane = nh * (int64_t)p->n * p->n;
+  switch (p->n) {
+#define CASE(N)                                                                   \
+  case N: {                                                                       \
+    int st;                                                                       \
+    if ((st = Launch<N>::green(c, p, spec, kern, kern_octant, work3))) return st; \
+    for (int d = 0; d < 3; ++d)                                                   \
+      if ((st = Launch<N>::c2c_y<+1>(c, p, work3 + d * plane))) return st;        \
+    if ((st = Launch<N>::c2r3(c, p, work3, force3, div, (unsigned *)c->ured))) return st; \
+    break;                                                                        \
+  }
+    FFT_SIZES(CASE)
+#undef CASE
+    default:
+      return CUBE_ERR_INVALID;
+  }
+  unsigned bits = 0;
+  CUBE_CHECK(hipMemcpyAsync(&bits, c->ured, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  float f;
+  memcpy(&f, &bits, sizeof(f));
+  if (f2max) *f2max = f;
+  return CUBE_OK;
+}

@@ -1,0 +1,140 @@
+/*
+ * cube_internal.h -- context layout and launch helpers of libcube_hip.so.
+ *
+ * Data layout in HBM (one image = the whole box on one GPU):
+ *  - particles: the reference's *disjoint* state only (no buffer copies);
+ *    cell-sorted in cell order cid = i + nt*(j + nt*(k + nt*tile)), tile =
+ *    tx + nnt*(ty + nnt*tz), with xp/vp as 3 interleaved izipx/izipv-byte
+ *    integers per particle and pid as int32;  cstart[cid] = first particle
+ *    of the cell (int64, exclusive prefix sum of rhoc).  A buffer cell of a
+ *    tile is read straight from the periodic neighbour's physical cell, so
+ *    buffer_grid/buffer_x/buffer_v reduce to index arithmetic.
+ *  - fine mesh: ONE periodic nf^3 mesh for the whole box (nf = 4 nc),
+ *    x fastest; its half spectrum (nf/2+1) x nf x nf complex.
+ *  - coarse mesh nc^3 and its half spectrum.
+ */
+#ifndef CUBE_INTERNAL_H
+#define CUBE_INTERNAL_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "../../include/cube.h"
+#include "cube_math.h"
+
+#define CUBE_CHECK(x)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) {                                                             \
+      fprintf(stderr, "cube: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return CUBE_ERR_HIP;                                                              \
+    }                                                                                   \
+  } while (0)
+#define CUBE_LAUNCH_CHECK() CUBE_CHECK(hipGetLastError())
+
+/* key of a particle inside its destination cell, see cube_drift.hip */
+#define KEY_PBITS 40
+#define KEY_PMASK ((1ull << KEY_PBITS) - 1ull)
+
+struct FFTPlan;  /* cube_fft.hip */
+
+struct cube_ctx {
+  cube_params p;
+  int device;
+  hipStream_t stream;
+  /* geometry */
+  int64_t nc, nnt, nt, ncb, ncell, nf, nh, ncells, ntiles;
+  int64_t np_image, np_image_max, np_tile_max, np_pp_max, cap;
+  int32_t nvbin;
+  /* particle state (disjoint, cell-sorted) and its double buffer */
+  void *xp, *vp, *xp2, *vp2;
+  int32_t *pid, *pid2;
+  int32_t *rhoc, *rhoc2;
+  float *vfield, *vfield2;
+  int64_t *cstart, *cstart2;  /* ncells+1 */
+  int64_t nplocal, npglobal;
+  int has_state;
+  /* drift scratch */
+  int32_t *dest;            /* cap: destination cell of each particle, -1 lost */
+  uint64_t *key;            /* cap: order key inside the destination cell      */
+  uint64_t *sortbuf;        /* cap + extra */
+  int32_t *fill;            /* ncells */
+  int32_t *xdest;           /* extra copies (duplicates in the reference)      */
+  uint64_t *xkey;
+  int64_t xcap;
+  /* codec tables */
+  float *tanf_tab;          /* nvbin: tanf(pi*code/(nvbin-1)), index code+nvbin/2 */
+  double *vdec0, *vdec1;    /* nvbin: decoded residual velocity for two sigmas  */
+  /* fine mesh */
+  float *rho_f;             /* nf^3 */
+  float2 *spec_f;           /* nh*nf*nf */
+  float2 *specd_f;          /* 3 * nh*nf*nf */
+  float *force_f;           /* 3*nf^3, (d fastest) */
+  float *kern_f;            /* 3*(nf/2+1)^3 octant */
+  /* coarse mesh */
+  float *rho_c;
+  float2 *spec_c, *specd_c;
+  float *force_c;           /* 3*nc^3 */
+  float *kern_c;            /* 3*nc*nc*(nc/2+1) */
+  int has_kernels;
+  /* small device scratch for reductions */
+  double *dred;             /* 1024 doubles */
+  uint64_t *ured;           /* 64 words */
+  int64_t *ired;            /* 4096 */
+  int64_t *scanbuf;         /* ncells/1024 + 2 block sums of the scan */
+  /* host-side persistent state */
+  float pp_ftemp;           /* running max of W/ext_pp_force.f90:123 */
+  FFTPlan *plan_f, *plan_c;
+};
+
+/* --- common device helpers --- */
+struct Geo {
+  int nt, nnt, nc, ncb;
+  __device__ __host__ int64_t cid(int i, int j, int k, int t) const {
+    return (int64_t)i + (int64_t)nt * ((int64_t)j + (int64_t)nt * ((int64_t)k + (int64_t)nt * t));
+  }
+};
+static inline Geo geo_of(const cube_ctx *c) {
+  Geo g;
+  g.nt = (int)c->nt; g.nnt = (int)c->nnt; g.nc = (int)c->nc; g.ncb = (int)c->ncb;
+  return g;
+}
+__device__ static inline int wrapi(int a, int n) {
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+/* cell id from global 0-based coarse coordinates (periodic) */
+__device__ static inline int64_t cid_global(const Geo &g, int gx, int gy, int gz) {
+  gx = wrapi(gx, g.nc); gy = wrapi(gy, g.nc); gz = wrapi(gz, g.nc);
+  const int t = gx / g.nt + g.nnt * (gy / g.nt + g.nnt * (gz / g.nt));
+  return g.cid(gx % g.nt, gy % g.nt, gz % g.nt, t);
+}
+
+/* loads/stores of 1- or 2-byte particle components */
+template <int B> struct IntOf;
+template <> struct IntOf<1> { typedef int8_t T; };
+template <> struct IntOf<2> { typedef int16_t T; };
+
+/* launchers (one file each) */
+int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi, cube_vstats *out);
+int cube_vdec_table(cube_ctx *c, double *dst, float sigma);
+int cube_scan_counts(cube_ctx *c, const int32_t *cnt, int64_t *start, int64_t n, int64_t *total);
+int cube_tile_boxsum(cube_ctx *c, const int32_t *rhoc, int w, int64_t *host_out);
+
+int cube_fft_create(cube_ctx *c, int n, FFTPlan **out);
+void cube_fft_destroy(FFTPlan *p);
+int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
+int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
+                          float2 *work3, float *force3, float scale_div, float *f2max);
+int cube_fft_forward_full(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
+int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out);
+
+int cube_fine_deposit(cube_ctx *c, float mass_p, double *testrhof);
+int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new);
+int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc);
+int cube_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax);
+int cube_green_init(cube_ctx *c, const float *wfxyzf, const float *wfxyzc);
+int cube_pp(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, float *f2_max_pp,
+            int64_t *itest1);
+int cube_vstats_run(cube_ctx *c, float sigma_vi, double *sums3);
+
+#endif

@@ -1,0 +1,469 @@
+/*
+ * cube_mesh.hip -- CIC deposits, force gathers (kicks) and the Green's
+ * functions of particle_mesh (W/pm.f90) and kernel_f/kernel_c
+ * (W/kernel_f.f90, W/kernel_c.f90) for a single image.
+ *
+ * Deposits are deterministic gathers: one thread per mesh cell walks the
+ * few source coarse cells that can reach it in the reference's own visit
+ * order (ilayer, k, j, i, l) with nlayer=3, re-evaluates each particle's
+ * real(4) CIC weights with the reference's expression in the owning tile's
+ * frame, and adds them in that order -- so every physical cell holds the
+ * bit pattern the reference's layered OpenMP loop produces for it.
+ */
+#include "cube_internal.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MT 256
+
+/* per-axis candidate source coarse cells of a mesh cell, in visit order
+   restricted to one axis: cells with different k are ordered by
+   (ilayer(k), k) with ilayer = (k - k0) mod 3 */
+struct Src {
+  int n;
+  int c[3];
+};
+__device__ static inline void order_z(Src &s, int k0) {
+  /* sort s.c[0..n) by ((c-k0) mod 3, c), n <= 3 */
+  for (int a = 1; a < s.n; ++a) {
+    const int x = s.c[a];
+    const int kx = ((x - k0) % 3 + 3) % 3;
+    int b = a - 1;
+    while (b >= 0) {
+      const int y = s.c[b];
+      const int ky = ((y - k0) % 3 + 3) % 3;
+      if (ky > kx || (ky == kx && y > x)) {
+        s.c[b + 1] = y;
+        --b;
+      } else {
+        break;
+      }
+    }
+    s.c[b + 1] = x;
+  }
+}
+
+/* ---------------- fine CIC deposit, W/pm.f90:50-108 ---------------- */
+template <int IX>
+__global__ __launch_bounds__(MT) void k_fine_deposit(const typename IntOf<IX>::T *__restrict__ xp,
+                                                     const int32_t *__restrict__ rhoc,
+                                                     const int64_t *__restrict__ cstart, Geo g, float mass,
+                                                     float *__restrict__ rho, double *__restrict__ part) {
+  const int nf = 4 * g.nc, nft = 4 * g.nt;
+  const int64_t nf3 = (int64_t)nf * nf * nf;
+  double tsum = 0;
+  for (int64_t f = blockIdx.x * (int64_t)MT + threadIdx.x; f < nf3; f += (int64_t)gridDim.x * MT) {
+    const int fx = (int)(f % nf), fy = (int)((f / nf) % nf), fz = (int)(f / ((int64_t)nf * nf));
+    const int fg[3] = {fx, fy, fz};
+    int T[3], lf[3];
+    Src s[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      T[a] = fg[a] / nft;
+      lf[a] = fg[a] - T[a] * nft + 1;            /* 1-based local fine index */
+      const int C = (lf[a] - 1) / 4 + 1, sub = (lf[a] - 1) % 4;
+      if (sub == 0) {
+        s[a].n = 2; s[a].c[0] = C - 1; s[a].c[1] = C;
+      } else {
+        s[a].n = 1; s[a].c[0] = C;
+      }
+    }
+    order_z(s[2], 2 - g.ncb);
+    float acc = 0.f;
+    for (int qz = 0; qz < s[2].n; ++qz)
+      for (int qy = 0; qy < s[1].n; ++qy)
+        for (int qx = 0; qx < s[0].n; ++qx) {
+          const int ci[3] = {s[0].c[qx], s[1].c[qy], s[2].c[qz]};
+          const int64_t cc = cid_global(g, T[0] * g.nt + ci[0] - 1, T[1] * g.nt + ci[1] - 1, T[2] * g.nt + ci[2] - 1);
+          const int np = rhoc[cc];
+          const int64_t p0 = cstart[cc];
+          for (int l = 0; l < np; ++l) {
+            float w[3];
+            bool hit = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const int32_t code = (int32_t)xp[3 * (p0 + l) + a];
+              const float tempx = (float)((double)(4 * (ci[a] - 1)) + 4.0 * cube_xoff(IX, code));
+              const int i1 = (int)floorf(tempx) + 1;
+              const float d1 = (float)i1 - tempx;
+              if (i1 == lf[a]) w[a] = d1;
+              else if (i1 + 1 == lf[a]) w[a] = 1.0f - d1;
+              else hit = false;
+            }
+            if (hit) acc = acc + w[0] * w[1] * w[2] * mass;
+          }
+        }
+    rho[f] = acc;
+    tsum += (double)acc;
+  }
+  __shared__ double red[MT];
+  red[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int o = MT / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+/* ---------------- coarse CIC deposit, W/pm.f90:205-267 ---------------- */
+template <int IX>
+__global__ __launch_bounds__(MT) void k_coarse_deposit(const typename IntOf<IX>::T *__restrict__ xp,
+                                                       const int32_t *__restrict__ rhoc,
+                                                       const int64_t *__restrict__ cstart, Geo g, float mass,
+                                                       float *__restrict__ r3, double *__restrict__ part) {
+  const int nc = g.nc;
+  const int64_t nc3 = (int64_t)nc * nc * nc;
+  double tsum = 0;
+  for (int64_t m = blockIdx.x * (int64_t)MT + threadIdx.x; m < nc3; m += (int64_t)gridDim.x * MT) {
+    const int gg[3] = {(int)(m % nc), (int)((m / nc) % nc), (int)(m / ((int64_t)nc * nc))};
+    int T[3], lm[3];
+    Src s[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      T[a] = gg[a] / g.nt;
+      lm[a] = gg[a] - T[a] * g.nt + 1;
+      s[a].n = 3; s[a].c[0] = lm[a] - 1; s[a].c[1] = lm[a]; s[a].c[2] = lm[a] + 1;
+    }
+    order_z(s[2], 0);
+    float acc = 0.f;
+    for (int qz = 0; qz < 3; ++qz)
+      for (int qy = 0; qy < 3; ++qy)
+        for (int qx = 0; qx < 3; ++qx) {
+          const int ci[3] = {s[0].c[qx], s[1].c[qy], s[2].c[qz]};
+          const int64_t cc = cid_global(g, T[0] * g.nt + ci[0] - 1, T[1] * g.nt + ci[1] - 1, T[2] * g.nt + ci[2] - 1);
+          const int np = rhoc[cc];
+          const int64_t p0 = cstart[cc];
+          for (int l = 0; l < np; ++l) {
+            float w[3];
+            bool hit = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const int32_t code = (int32_t)xp[3 * (p0 + l) + a];
+              const float tempx = (float)(((double)(ci[a] - 1) + cube_xoff(IX, code)) - 0.5);
+              const int i1 = (int)floorf(tempx) + 1;
+              const float d1 = (float)i1 - tempx;
+              if (i1 == lm[a]) w[a] = d1;
+              else if (i1 + 1 == lm[a]) w[a] = 1.0f - d1;
+              else hit = false;
+            }
+            if (hit) acc = acc + w[0] * w[1] * w[2] * mass;
+          }
+        }
+    r3[m] = acc;
+    tsum += (double)acc;
+  }
+  __shared__ double red[MT];
+  red[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int o = MT / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_parts1(const double *part, int nb, double *out) {
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int b = 0; b < nb; ++b) s += part[b];
+    *out = s;
+  }
+}
+
+#define DEP_BLOCKS 2048
+template <int IX> static int fine_dep(cube_ctx *c, float mass, double *t) {
+  hipLaunchKernelGGL(k_fine_deposit<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
+                     (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_f, c->dred);
+  CUBE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, c->dred, DEP_BLOCKS, c->dred + 4000);
+  CUBE_LAUNCH_CHECK();
+  if (t) {
+    CUBE_CHECK(hipMemcpyAsync(t, c->dred + 4000, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CUBE_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return CUBE_OK;
+}
+int cube_fine_deposit(cube_ctx *c, float mass_p, double *testrhof) {
+  return c->p.izipx == 1 ? fine_dep<1>(c, mass_p, testrhof) : fine_dep<2>(c, mass_p, testrhof);
+}
+template <int IX> static int coarse_dep(cube_ctx *c, float mass, double *t) {
+  hipLaunchKernelGGL(k_coarse_deposit<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
+                     (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, c->dred);
+  CUBE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, c->dred, DEP_BLOCKS, c->dred + 4000);
+  CUBE_LAUNCH_CHECK();
+  if (t) {
+    CUBE_CHECK(hipMemcpyAsync(t, c->dred + 4000, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CUBE_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return CUBE_OK;
+}
+int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc) {
+  return c->p.izipx == 1 ? coarse_dep<1>(c, mass_p, testrhoc) : coarse_dep<2>(c, mass_p, testrhoc);
+}
+
+/* ---------------- kicks ---------------- */
+/* one workgroup per cell row (tile, k, j): particles of the row are
+   contiguous; each thread finds its cell by binary search in LDS */
+#define KT 256
+
+template <int IX, int IV, bool FINE>
+__global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__restrict__ xp,
+                                             typename IntOf<IV>::T *__restrict__ vp,
+                                             const int64_t *__restrict__ cstart, const float *__restrict__ vfield,
+                                             const float *__restrict__ force, const double *__restrict__ vdec,
+                                             Geo g, int nvh, float a_mid, float dt, double venc,
+                                             unsigned long long *__restrict__ vmax_bits) {
+  const int nt = g.nt;
+  const int64_t row = blockIdx.x;
+  const int j = (int)(row % nt), k = (int)((row / nt) % nt), t = (int)(row / ((int64_t)nt * nt));
+  const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
+  __shared__ int64_t cs[520];
+  const int64_t c0 = row * nt;
+  for (int i = threadIdx.x; i <= nt; i += KT) cs[i] = cstart[c0 + i];
+  __syncthreads();
+  const int nfine = 4 * g.nc, nft = 4 * nt;
+  double vm[3] = {0, 0, 0};
+  for (int64_t p = cs[0] + threadIdx.x; p < cs[nt]; p += KT) {
+    int lo = 0, hi = nt;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (cs[mid] <= p) lo = mid; else hi = mid;
+    }
+    const int loc[3] = {lo + 1, j + 1, k + 1};
+    const int tt[3] = {tx, ty, tz};
+    int i1[3];
+    float d1[3], d2[3];
+    double vreal[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int32_t code = (int32_t)xp[3 * p + a];
+      float tempx;
+      if (FINE) /* ncell*((/i,j,k/)-1)+ncell*(...)*x_resolution, tile frame (W/pm.f90:138) */
+        tempx = (float)((double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, code));
+      else /* ((/itx,../)-1)*nt+((/i,../)-1)+(...)*x_resolution-0.5 (W/pm.f90:335) */
+        tempx = (float)(((double)(tt[a] * nt + (loc[a] - 1)) + cube_xoff(IX, code)) - 0.5);
+      i1[a] = (int)floorf(tempx) + 1;
+      d1[a] = (float)i1[a] - tempx;
+      d2[a] = 1.0f - d1[a];
+      vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
+    }
+    /* 0-based periodic mesh coordinates of the two CIC planes per axis */
+    int m1[3], m2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (FINE) {
+        m1[a] = wrapi(tt[a] * nft + i1[a] - 1, nfine);
+        m2[a] = wrapi(tt[a] * nft + i1[a], nfine);
+      } else {
+        m1[a] = wrapi(i1[a] - 1, g.nc);
+        m2[a] = wrapi(i1[a], g.nc);
+      }
+    }
+    const int n = FINE ? nfine : g.nc;
+    /* corner order of W/pm.f90:146-153 */
+    const int cx[8] = {0, 1, 0, 0, 0, 1, 1, 1}, cy[8] = {0, 0, 1, 0, 1, 0, 1, 1}, cz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int X = cx[q] ? m2[0] : m1[0], Y = cy[q] ? m2[1] : m1[1], Z = cz[q] ? m2[2] : m1[2];
+      const float wx = cx[q] ? d2[0] : d1[0], wy = cy[q] ? d2[1] : d1[1], wz = cz[q] ? d2[2] : d1[2];
+      const float *F = force + 3 * (((int64_t)Z * n + Y) * n + X);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float inc = ((((((F[a] * a_mid) * dt) / 6.0f) / CUBE_PI_F) * wx) * wy) * wz;
+        vreal[a] = vreal[a] + (double)inc;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (!FINE) {
+        const double av = fabs(vreal[a] + (double)vfield[3 * (c0 + lo) + a]);
+        vm[a] = av > vm[a] ? av : vm[a];
+      }
+      vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal[a]);
+    }
+  }
+  if (!FINE) {
+    __shared__ double red[3][KT];
+    for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = vm[a];
+    __syncthreads();
+    for (int o = KT / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o)
+        for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = fmax(red[a][threadIdx.x], red[a][threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x < 3) atomicMax(vmax_bits + threadIdx.x, (unsigned long long)__double_as_longlong(red[threadIdx.x][0]));
+  }
+}
+
+template <int IX, int IV, bool FINE>
+static int kick_impl(cube_ctx *c, const float *force, float a_mid, float dt, double venc,
+                     unsigned long long *vmb) {
+  const int64_t nrows = c->ncells / c->nt;
+  hipLaunchKernelGGL((k_kick<IX, IV, FINE>), dim3((unsigned)nrows), dim3(KT), 0, c->stream,
+                     (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->cstart, c->vfield, force,
+                     c->vdec0, geo_of(c), c->nvbin / 2, a_mid, dt, venc, vmb);
+  CUBE_LAUNCH_CHECK();
+  return CUBE_OK;
+}
+template <bool FINE>
+static int kick(cube_ctx *c, const float *force, float a_mid, float dt, double venc, unsigned long long *vmb) {
+  const int x = c->p.izipx, v = c->p.izipv;
+  if (x == 1 && v == 1) return kick_impl<1, 1, FINE>(c, force, a_mid, dt, venc, vmb);
+  if (x == 1) return kick_impl<1, 2, FINE>(c, force, a_mid, dt, venc, vmb);
+  if (v == 1) return kick_impl<2, 1, FINE>(c, force, a_mid, dt, venc, vmb);
+  return kick_impl<2, 2, FINE>(c, force, a_mid, dt, venc, vmb);
+}
+
+/* fine kick, W/pm.f90:129-159: decode with sigma_vi, encode with sigma_vi_new */
+int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new) {
+  int st = cube_vdec_table(c, c->vdec0, sigma_vi);
+  if (st) return st;
+  st = kick<true>(c, c->force_f, a_mid, dt, cube_vconst(sigma_vi_new), nullptr);
+  if (st) return st;
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  return CUBE_OK;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+/* coarse kick, W/pm.f90:322-387: decode and encode with sigma_vi */
+int cube_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax) {
+  int st = cube_vdec_table(c, c->vdec0, sigma_vi);
+  if (st) return st;
+  unsigned long long *vmb = (unsigned long long *)c->ured + 8;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, vmb, 3);
+  CUBE_LAUNCH_CHECK();
+  st = kick<false>(c, c->force_c, a_mid, dt, cube_vconst(sigma_vi), vmb);
+  if (st) return st;
+  unsigned long long h[3];
+  CUBE_CHECK(hipMemcpyAsync(h, vmb, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  for (int a = 0; a < 3; ++a) {
+    double d;
+    memcpy(&d, &h[a], sizeof(d));
+    if (vmax) vmax[a] = (float)d; /* vmax is real(4) (W/variables.f90:35) */
+  }
+  return CUBE_OK;
+}
+
+/* ---------------- Green's functions ---------------- */
+/* real-space fine kernel of component d on the nf^3 mesh: the 16^3 table at
+   offsets 0..15, odd-mirrored in axis d and even in the others
+   (W/kernel_f.f90:25-31), corner |o|<=pp_range zeroed when PP is on */
+__global__ void k_fine_kernel_rs(float *rs, int n, const float *tab, int d, int pp_on, int pp_range) {
+  const int64_t n3 = (int64_t)n * n * n;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n3; q += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(q % n), y = (int)((q / n) % n), z = (int)(q / ((int64_t)n * n));
+    const int o[3] = {x <= n / 2 ? x : x - n, y <= n / 2 ? y : y - n, z <= n / 2 ? z : z - n};
+    float v = 0.f;
+    if (o[0] >= -15 && o[0] <= 15 && o[1] >= -15 && o[1] <= 15 && o[2] >= -15 && o[2] <= 15) {
+      const int a0 = o[0] < 0 ? -o[0] : o[0], a1 = o[1] < 0 ? -o[1] : o[1], a2 = o[2] < 0 ? -o[2] : o[2];
+      const bool zero = pp_on && a0 <= pp_range && a1 <= pp_range && a2 <= pp_range;
+      if (!zero) {
+        v = tab[((a2 * 16 + a1) * 16 + a0) * 3 + d];
+        if (o[d] < 0) v = -v;
+      }
+    }
+    rs[q] = v;
+  }
+}
+/* keep Im of the positive-frequency octant: kern[d][kz][ky][kx], 0..n/2 */
+__global__ void k_extract_octant(const float2 *spec, int n, float *kern, int d) {
+  const int h = n / 2 + 1, nh = n / 2 + 1;
+  const int64_t tot = (int64_t)h * h * h;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(q % h), ky = (int)((q / h) % h), kz = (int)(q / ((int64_t)h * h));
+    kern[(int64_t)d * tot + q] = spec[((int64_t)kz * n + ky) * nh + kx].y;
+  }
+}
+/* uncorrected coarse kernel -r/r^3 in fine units (W/kernel_c.f90:16-28) and
+   the table-corrected one (:31-72), component d */
+__global__ void k_coarse_kernel_rs(float *rs_plain, float *rs_corr, int nc, const float *ctab, int d) {
+  const int64_t n3 = (int64_t)nc * nc * nc;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n3; q += (int64_t)gridDim.x * blockDim.x) {
+    const int ii[3] = {(int)(q % nc), (int)((q / nc) % nc), (int)(q / ((int64_t)nc * nc))};
+    float rx[3];
+    for (int a = 0; a < 3; ++a) rx[a] = (float)(4 * ((ii[a] + nc / 2) % nc - nc / 2));
+    const float r = sqrtf((rx[0] * rx[0] + rx[1] * rx[1]) + rx[2] * rx[2]);
+    const float v = r == 0.f ? 0.f : -rx[d] / (r * (r * r));
+    rs_plain[q] = v;
+    /* octant copies of the 4^3 table: low block 0..3 or high block nc-3..nc-1
+       (mirrored index nc-i), sign flipped for component d in a high block */
+    int ti[3];
+    bool in = true;
+    float sgn = 1.f;
+    for (int a = 0; a < 3; ++a) {
+      if (ii[a] < 4) ti[a] = ii[a];
+      else if (ii[a] >= nc - 3) {
+        ti[a] = nc - ii[a];
+        if (a == d) sgn = -sgn;
+      } else in = false;
+    }
+    rs_corr[q] = in ? sgn * ctab[((ti[2] * 4 + ti[1]) * 4 + ti[0]) * 3 + d] : v;
+  }
+}
+/* LRCKCORR (W/kernel_c.f90:301-325), all real(4) */
+__global__ void k_lrckcorr(const float2 *spec_corr, const float2 *spec_plain, int nc, float *kern, int d) {
+  const int nh = nc / 2 + 1;
+  const int64_t tot = (int64_t)nh * nc * nc;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(q % nh), iy = (int)((q / nh) % nc), iz = (int)(q / ((int64_t)nh * nc));
+    const int ki[3] = {(ix + nc / 2) % nc - nc / 2, (iy + nc / 2) % nc - nc / 2, (iz + nc / 2) % nc - nc / 2};
+    float kx[3], ks[3];
+    for (int a = 0; a < 3; ++a) {
+      kx[a] = (float)ki[a];
+      ks[a] = 2.0f * sinf(CUBE_PI_F * kx[a] / (float)nc);
+    }
+    const float kr = sqrtf((kx[0] * kx[0] + kx[1] * kx[1]) + kx[2] * kx[2]);
+    const float kc = spec_corr[q].y;
+    const float cu = spec_plain[q].y;
+    float out = kc;
+    if (!(kr > 8.0f || ki[d] == 0)) {
+      const float ssum = (ks[0] * ks[0] + ks[1] * ks[1]) + ks[2] * ks[2];
+      out = ((((kc * 0.25f) * CUBE_PI_F) * ks[d]) / ssum) / cu;
+    }
+    kern[(int64_t)d * tot + q] = out;
+  }
+}
+
+int cube_green_init(cube_ctx *c, const float *wfxyzf, const float *wfxyzc) {
+  float *dtab = nullptr, *dctab = nullptr;
+  CUBE_CHECK(hipMalloc(&dtab, 16 * 16 * 16 * 3 * sizeof(float)));
+  CUBE_CHECK(hipMalloc(&dctab, 4 * 4 * 4 * 3 * sizeof(float)));
+  CUBE_CHECK(hipMemcpy(dtab, wfxyzf, 16 * 16 * 16 * 3 * sizeof(float), hipMemcpyHostToDevice));
+  CUBE_CHECK(hipMemcpy(dctab, wfxyzc, 4 * 4 * 4 * 3 * sizeof(float), hipMemcpyHostToDevice));
+  int st = CUBE_OK;
+  const int nf = (int)c->nf, nc = (int)c->nc;
+  for (int d = 0; d < 3 && !st; ++d) {
+    hipLaunchKernelGGL(k_fine_kernel_rs, dim3(2048), dim3(256), 0, c->stream, c->rho_f, nf, dtab, d, c->p.pp_on,
+                       c->p.pp_range);
+    if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
+    if (!st) st = cube_fft_forward_full(c, c->plan_f, c->rho_f, c->spec_f);
+    if (!st) {
+      hipLaunchKernelGGL(k_extract_octant, dim3(2048), dim3(256), 0, c->stream, c->spec_f, nf, c->kern_f, d);
+      if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
+    }
+  }
+  for (int d = 0; d < 3 && !st; ++d) {
+    float *plain = c->rho_c, *corr = (float *)c->specd_c; /* specd_c is large enough for nc^3 floats */
+    float2 *sp_plain = c->spec_c, *sp_corr = c->specd_c + (nc / 2 + 1) * (int64_t)nc * nc;
+    hipLaunchKernelGGL(k_coarse_kernel_rs, dim3(1024), dim3(256), 0, c->stream, plain, corr, nc, dctab, d);
+    if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
+    if (!st) st = cube_fft_forward_full(c, c->plan_c, plain, sp_plain);
+    /* corr lives in specd_c plane 0; transform it into plane 1 */
+    if (!st) st = cube_fft_forward_full(c, c->plan_c, corr, sp_corr);
+    if (!st) {
+      hipLaunchKernelGGL(k_lrckcorr, dim3(1024), dim3(256), 0, c->stream, sp_corr, sp_plain, nc, c->kern_c, d);
+      if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
+    }
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dtab);
+  (void)hipFree(dctab);
+  return st;
+}

@@ -1,0 +1,168 @@
+/*
+ * cube_pp.hip -- ext_pp_force (W/ext_pp_force.f90:10-162) on the whole box.
+ *
+ * The reference builds, per tile, a head-of-chain linked list over fine
+ * cells (hoc/ll, :53-75): particles of coarse cells 0..nt+1 are pushed in
+ * (igz,igy,igx,lp) order, so a fine cell's chain lists its particles --
+ * which all sit in one coarse cell -- in decreasing l.  Each physical
+ * particle then sums the FP64 pair force over the (2*pp_range+1)^3
+ * neighbouring fine cells in (kk,jj,ii) order, chain order inside a cell.
+ *
+ * Here the chain is implicit: the particles of a fine cell are those of its
+ * coarse cell whose position code has the fine sub-cell index (top two bits
+ * of the code for ncell=4), visited from the last particle of the coarse
+ * cell backwards.  One thread per particle sums in exactly the reference
+ * order, so the FP64 sums and the re-encoded velocities are bit-identical.
+ */
+#include "cube_internal.h"
+#include <math.h>
+#include <string.h>
+
+#define PT 256
+
+__device__ static inline double powi3(double x) { return x * (x * x); }
+__device__ static inline double powi5(double x) {
+  const double x2 = x * x;
+  return x2 * (x * x2);
+}
+
+template <int IX, int IV>
+__global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restrict__ xp,
+                                           typename IntOf<IV>::T *__restrict__ vp,
+                                           const int32_t *__restrict__ rhoc, const int64_t *__restrict__ cstart,
+                                           const double *__restrict__ vdec, Geo g, int nvh, int pr, double rsoft,
+                                           double mass, double a_mid, double dt, double venc,
+                                           unsigned long long *__restrict__ out) {
+  const int nt = g.nt;
+  const int64_t row = blockIdx.x;
+  const int j = (int)(row % nt), k = (int)((row / nt) % nt), t = (int)(row / ((int64_t)nt * nt));
+  const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
+  const int tt[3] = {tx, ty, tz};
+  __shared__ int64_t cs[520];
+  const int64_t c0 = row * nt;
+  for (int i = threadIdx.x; i <= nt; i += PT) cs[i] = cstart[c0 + i];
+  __syncthreads();
+  const int sh = IX == 1 ? 6 : 14; /* fine sub-cell = code >> sh (ncell=4) */
+  double f2m = 0;
+  unsigned long long ntest = 0;
+  for (int64_t p = cs[0] + threadIdx.x; p < cs[nt]; p += PT) {
+    int lo = 0, hi = nt;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (cs[mid] <= p) lo = mid; else hi = mid;
+    }
+    const int loc[3] = {lo + 1, j + 1, k + 1};
+    double x1[3], vreal[3], ft[3] = {0, 0, 0};
+    int gf[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int32_t code = (int32_t)xp[3 * p + a];
+      /* xvec1 = ncell*floor((ig-1)/4.) + ncell*(...)*x_resolution with ig the
+         particle's fine cell; floor((ig-1)/4.) = coarse cell - 1 */
+      x1[a] = (double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, code);
+      gf[a] = 4 * (loc[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1; /* fine cell, 1-based, tile frame */
+      vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
+    }
+    for (int kk = gf[2] - pr; kk <= gf[2] + pr; ++kk)
+      for (int jj = gf[1] - pr; jj <= gf[1] + pr; ++jj)
+        for (int ii = gf[0] - pr; ii <= gf[0] + pr; ++ii) {
+          const int fc[3] = {ii, jj, kk};
+          int cc[3], sub[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            cc[a] = (int)floorf((float)(fc[a] - 1) / 4.0f) + 1; /* coarse cell, tile frame */
+            sub[a] = (fc[a] - 1) - 4 * (cc[a] - 1);
+          }
+          const int64_t cq = cid_global(g, tt[0] * nt + cc[0] - 1, tt[1] * nt + cc[1] - 1, tt[2] * nt + cc[2] - 1);
+          const int np = rhoc[cq];
+          const int64_t q0 = cstart[cq];
+          for (int l = np - 1; l >= 0; --l) {
+            const int64_t q = q0 + l;
+            const int32_t c0x = (int32_t)xp[3 * q], c1x = (int32_t)xp[3 * q + 1], c2x = (int32_t)xp[3 * q + 2];
+            if ((int)(cube_ucode(IX, c0x) >> sh) != sub[0] || (int)(cube_ucode(IX, c1x) >> sh) != sub[1] ||
+                (int)(cube_ucode(IX, c2x) >> sh) != sub[2])
+              continue;
+            const int32_t cd[3] = {c0x, c1x, c2x};
+            double x21[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) x21[a] = ((double)(4 * (cc[a] - 1)) + 4.0 * cube_xoff(IX, cd[a])) - x1[a];
+            double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
+            if (rmag == 0.0) rmag = 1.0;
+            const double rcut = rmag / 16.0;
+            const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
+            const double r3 = powi3(rmag);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              double fp = (mass * (x21[a] / r3)) * pcut;
+              if (!(rmag > rsoft)) fp = fp * 0.0;
+              ft[a] = ft[a] + fp;
+            }
+          }
+        }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      vreal[a] = vreal[a] + (((ft[a] * a_mid) * dt) / 6.0) / (double)CUBE_PI_F;
+      vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal[a]);
+    }
+    const double f2 = (ft[0] * ft[0] + ft[1] * ft[1]) + ft[2] * ft[2];
+    f2m = f2 > f2m ? f2 : f2m;
+    ++ntest;
+  }
+  __shared__ double red[PT];
+  __shared__ unsigned long long cnt[PT];
+  red[threadIdx.x] = f2m;
+  cnt[threadIdx.x] = ntest;
+  __syncthreads();
+  for (int o = PT / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+      cnt[threadIdx.x] += cnt[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+    atomicAdd(out + 1, cnt[0]);
+  }
+}
+
+__global__ void k_zero_u64_pp(unsigned long long *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+template <int IX, int IV>
+static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, unsigned long long *out) {
+  const int64_t nrows = c->ncells / c->nt;
+  hipLaunchKernelGGL((k_pp<IX, IV>), dim3((unsigned)nrows), dim3(PT), 0, c->stream,
+                     (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
+                     c->vdec0, geo_of(c), c->nvbin / 2, (int)c->p.pp_range, (double)c->p.rsoft, (double)mass_p,
+                     (double)a_mid, (double)dt, cube_vconst(sigma_vi), out);
+  CUBE_LAUNCH_CHECK();
+  return CUBE_OK;
+}
+
+int cube_pp(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, float *f2_max_pp, int64_t *itest1) {
+  int st = cube_vdec_table(c, c->vdec0, sigma_vi);
+  if (st) return st;
+  unsigned long long *out = (unsigned long long *)c->ured + 16;
+  hipLaunchKernelGGL(k_zero_u64_pp, dim3(1), dim3(64), 0, c->stream, out, 2);
+  CUBE_LAUNCH_CHECK();
+  const int x = c->p.izipx, v = c->p.izipv;
+  if (x == 1 && v == 1) st = pp_impl<1, 1>(c, a_mid, dt, sigma_vi, mass_p, out);
+  else if (x == 1) st = pp_impl<1, 2>(c, a_mid, dt, sigma_vi, mass_p, out);
+  else if (v == 1) st = pp_impl<2, 1>(c, a_mid, dt, sigma_vi, mass_p, out);
+  else st = pp_impl<2, 2>(c, a_mid, dt, sigma_vi, mass_p, out);
+  if (st) return st;
+  unsigned long long h[2];
+  CUBE_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  double f2;
+  memcpy(&f2, &h[0], sizeof(f2));
+  /* ftemp=max(ftemp,sum(f_tot**2)) with a `save`d real(4) ftemp that is
+     never reset (W/ext_pp_force.f90:20,123): a running max over calls */
+  const double prev = (double)c->pp_ftemp;
+  c->pp_ftemp = (float)(f2 > prev ? f2 : prev);
+  if (f2_max_pp) *f2_max_pp = c->pp_ftemp;
+  if (itest1) *itest1 = (int64_t)h[1];
+  return CUBE_OK;
+}

@@ -1,0 +1,166 @@
+/*
+ * cube.h -- C ABI of the MI355X CUBE particle-mesh step (libcube_hip.so).
+ *
+ * The reference has no FFI: its hot path is a set of parameterless Fortran
+ * subroutines that mutate module state, called from the step loop of
+ * CUBEnu/work/main/main.f90:102-118.  Each entry point below replaces one of
+ * them for a single image (nn=1) covering the whole periodic box on one GPU,
+ * keeping the call order and argument meaning:
+ *
+ *   call timestep       -> host (FP64 a(t)), W/timestep.f90:1-92
+ *   call update_x       -> cube_update_x        (W/update_particle.f90:5-256)
+ *   call buffer_grid    -> cube_buffer_grid     (W/buffer_grid.f90:5-209)
+ *   call buffer_x       -> cube_buffer_x        (W/buffer_particle.f90:5-14)
+ *   call ext_pp_force   -> cube_pp_force        (W/ext_pp_force.f90:10-162)
+ *   call particle_mesh  -> cube_particle_mesh   (W/pm.f90:2-415)
+ *   call buffer_v       -> cube_buffer_v        (W/buffer_particle.f90:16-25)
+ *   call kernel_f/_c    -> cube_init_kernels    (W/kernel_f.f90, W/kernel_c.f90)
+ *   particle_initialization / checkpoint
+ *                       -> cube_set_state / cube_get_state
+ *                          (W/particle_initialization.f90:34-82,
+ *                           W/checkpoint.f90:38-44)
+ * (W/ = CUBEnu/work/main/.)
+ *
+ * Conventions
+ *  - Every entry point returns int: 0 = success, otherwise a CUBE_ERR_* code;
+ *    each reference `stop` condition has its own code.  No exception or
+ *    longjmp crosses the ABI.
+ *  - The context owns all device memory; host buffers are copied, never kept.
+ *  - Arrays use the reference's checkpoint layout (Fortran column-major):
+ *      xp  : izipx-byte signed integers, 3 per particle (x fastest)
+ *      vp  : izipv-byte signed integers, 3 per particle
+ *      pid : int32 per particle (optional)
+ *      rhoc: int32 per coarse cell, order (i,j,k,tile_x,tile_y,tile_z)
+ *      vfield: 3 float per coarse cell, same cell order
+ *    Particles are cell-sorted in that cell order.
+ *  - Calls are synchronous for their scalar outputs; one host thread per
+ *    context; not re-entrant.  Particle counts and indices are int64.
+ */
+#ifndef CUBE_H
+#define CUBE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CUBE_OK = 0,
+  CUBE_ERR_TILE_OVERFLOW = 1,  /* W/update_particle.f90:88-94   tile_buffer too small  */
+  CUBE_ERR_IMAGE_OVERFLOW = 2, /* W/buffer_grid.f90:148-154     image_buffer too small */
+  CUBE_ERR_PP_OVERFLOW = 3,    /* W/ext_pp_force.f90:41-45      np_pp_max too small    */
+  CUBE_ERR_CHECKSUM = 4,       /* W/buffer_grid.f90:174-177     particle checksum      */
+  CUBE_ERR_COUNT = 5,          /* W/update_particle.f90:245-250, W/ext_pp_force.f90:156-160 */
+  CUBE_ERR_DISPLACEMENT = 6,   /* a particle moved past the buffer (rhoce bounds)     */
+  CUBE_ERR_HIP = 10,           /* a HIP runtime call failed                           */
+  CUBE_ERR_ALLOC = 11,         /* device allocation failed                            */
+  CUBE_ERR_INVALID = 12,       /* bad argument or unsupported size                    */
+  CUBE_ERR_STATE = 13          /* call out of order (no state / no kernels)           */
+};
+
+/* Compile-time constants of W/parameters.f90:8-123 that the path reads. */
+typedef struct {
+  int64_t nc;        /* coarse cells per dim of the (single) image       */
+  int64_t nnt;       /* tiles per dim (fixes the reference visit order)   */
+  int64_t ncb;       /* buffer depth in coarse cells (6)                  */
+  int64_t np_nc;     /* particles per coarse cell per dim (np_image)      */
+  int32_t izipx;     /* bytes per position component: 1 or 2             */
+  int32_t izipv;     /* bytes per velocity component: 1 or 2             */
+  int32_t has_pid;   /* carry particle ids                               */
+  int32_t pp_on;     /* Extended_pp_force                                 */
+  int32_t pp_range;  /* PP neighbour range in fine cells (<=4)            */
+  float rsoft;       /* PP softening length (fine cells)                  */
+  float image_buffer;/* reference image_buffer (1.5)                      */
+  float tile_buffer; /* reference tile_buffer (2.5)                       */
+  int32_t device;    /* HIP device ordinal                                */
+  int32_t reserved;
+} cube_params;
+
+/* Outputs of update_x (W/update_particle.f90:172-250). */
+typedef struct {
+  double sum_c, sum_r, sum_s;              /* FP64 sums of v_c^2, v_res^2, v^2   */
+  double std_vsim, std_vsim_c, std_vsim_res;
+  float sigma_vi_new;                      /* std_vsim_res/sqrt(3.), real(4)     */
+  int32_t nlayer;                          /* 2*ceiling(dt_mid*vz_max/ncell)+1   */
+  int64_t nplocal;                         /* particles after the drift          */
+  int64_t lost;                            /* particles that left every tile     */
+  double overhead_tile;                    /* max tile fill / np_tile_max        */
+} cube_vstats;
+
+/* Outputs of particle_mesh (W/pm.f90:391-410). */
+typedef struct {
+  float f2_max_fine, f2_max_coarse;
+  float dt_fine, dt_coarse, dt_vmax, vz_max;
+  float vmax[3];
+  float reserved;
+  double testrhof, testrhoc;               /* total fine / coarse mass          */
+} cube_pm_out;
+
+typedef struct cube_ctx cube_ctx;
+
+/* Context owning device memory, sized from the parameters. */
+int cube_ctx_create(const cube_params *p, cube_ctx **out);
+void cube_ctx_destroy(cube_ctx *ctx);
+const char *cube_strerror(int code);
+/* ints[0..15]: nt, nf, nh, np_image, np_image_max, np_tile_max, np_pp_max,
+   capacity, nplocal, npglobal, ... ; used by the bindings */
+int cube_ctx_info(const cube_ctx *ctx, int64_t *ints16);
+
+/* kernel_f + kernel_c (W/initialize.f90:37-38).  wfxyzf: 16*16*16*3 floats
+   of CUBEnu/kernels/wfxyzf.3.ascii, order (i,j,k) with i fastest, 3
+   components per entry; wfxyzc: 4*4*4*3 floats of wfxyzc.2.ascii. */
+int cube_init_kernels(cube_ctx *ctx, const float *wfxyzf, const float *wfxyzc);
+
+/* particle_initialization: disjoint state in checkpoint layout. */
+int cube_set_state(cube_ctx *ctx, int64_t nplocal, int64_t npglobal, const void *xp,
+                   const void *vp, const int32_t *pid, const int32_t *rhoc,
+                   const float *vfield);
+/* checkpoint: copy the disjoint state back (any pointer may be NULL). */
+int cube_get_state(cube_ctx *ctx, int64_t *nplocal, void *xp, void *vp, int32_t *pid,
+                   int32_t *rhoc, float *vfield);
+
+/* The step, in W/main.f90:102-118 order. */
+int cube_update_x(cube_ctx *ctx, float dt_old, float dt, float vz_max, float sigma_vi,
+                  cube_vstats *out);
+int cube_buffer_grid(cube_ctx *ctx);
+int cube_buffer_x(cube_ctx *ctx);
+int cube_pp_force(cube_ctx *ctx, float a_mid, float dt, float sigma_vi, float mass_p,
+                  float *f2_max_pp);
+int cube_particle_mesh(cube_ctx *ctx, float a_mid, float dt, float sigma_vi,
+                       float sigma_vi_new, float mass_p, cube_pm_out *out);
+int cube_buffer_v(cube_ctx *ctx);
+
+/* particle_mesh split into its stages (W/pm.f90:49-126, 129-159, 202-267,
+   271-315, 319-387), for callers that interleave their own work and for
+   per-stage parity checks. */
+int cube_pm_fine_deposit(cube_ctx *ctx, float mass_p, double *testrhof);
+int cube_pm_fine_force(cube_ctx *ctx, float *f2_max_fine);
+int cube_pm_fine_kick(cube_ctx *ctx, float a_mid, float dt, float sigma_vi,
+                      float sigma_vi_new);
+int cube_pm_coarse_deposit(cube_ctx *ctx, float mass_p, double *testrhoc);
+int cube_pm_coarse_force(cube_ctx *ctx, float *f2_max_coarse);
+int cube_pm_coarse_kick(cube_ctx *ctx, float a_mid, float dt, float sigma_vi, float *vmax);
+
+/* Mesh fields, for diagnostics and checks.  which: */
+enum {
+  CUBE_FIELD_RHO_F = 0,   /* nf^3 float, fine density, (x,y,z) x fastest           */
+  CUBE_FIELD_FORCE_F = 1, /* nf^3 x 3 float, fine force, 3 components per cell      */
+  CUBE_FIELD_RHO_C = 2,   /* nc^3 float, coarse density                             */
+  CUBE_FIELD_FORCE_C = 3, /* nc^3 x 3 float, coarse force                           */
+  CUBE_FIELD_KERN_F = 4,  /* 3 x (nf/2+1)^3 float, fine Green's function octant     */
+  CUBE_FIELD_KERN_C = 5   /* 3 x nc x nc x (nc/2+1) float, coarse Green's function  */
+};
+int cube_get_field(cube_ctx *ctx, int which, void *host, size_t bytes);
+int cube_set_field(cube_ctx *ctx, int which, const void *host, size_t bytes);
+
+/* Wait for all work of the context; returns CUBE_ERR_HIP on a device error. */
+int cube_sync(cube_ctx *ctx);
+
+/* Host-side twins of device math, exported for the bindings' checks. */
+double cube_host_atan(double x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+/*
+ * cube_oracle.h -- CPU restatement of the CUBE particle-mesh step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity oracle for the HIP library in
+ * cafproject_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker / CPU baseline, never
+ * as a product code path.
+ *
+ * It follows the reference Fortran loop for loop, with the reference's
+ * buffered ("extended tile") particle layout, single image (nn=1), so every
+ * coarray reference is a periodic self-copy:
+ *   update_xp ........ CUBEnu/work/main/update_particle.f90:16-256
+ *   spine_tile/image . CUBEnu/work/main/variables.f90:96-211
+ *   buffer_grid ...... CUBEnu/work/main/buffer_grid.f90:5-209
+ *   buffer_xvp ....... CUBEnu/work/main/buffer_particle.f90:27-357
+ *   ext_pp_force ..... CUBEnu/work/main/ext_pp_force.f90:10-162
+ *   particle_mesh .... CUBEnu/work/main/pm.f90:2-415 (particle loops here;
+ *                      the FFTs are done by oracle/oracle.py with numpy/scipy)
+ * Fortran column-major arrays are kept with their Fortran bounds; index
+ * helpers below map them to flat C arrays.
+ */
+#ifndef CUBE_ORACLE_H
+#define CUBE_ORACLE_H
+#include <stdint.h>
+
+typedef struct {
+  /* geometry, W/parameters.f90:27-54 */
+  int64_t nc, nnt, nt, ncb, ncell, nte, nft, nfb, nfe, nf_cutoff;
+  int64_t izipx, izipv;          /* bytes per x / v component (1 or 2) */
+  int64_t np_nc;                 /* particles per coarse cell per dim */
+  int64_t pp_on, pp_range;       /* Extended_pp_force, pp_range */
+  float rsoft, image_buffer, tile_buffer;
+  int64_t has_pid;
+  /* derived, W/variables.f90:8-10 */
+  int64_t np_image, np_image_max, np_tile_max, np_pp_max;
+  int64_t nvbin;
+  /* state */
+  int16_t *xp;     /* 3 x np_image_max, value range of izipx bytes */
+  int16_t *vp;     /* 3 x np_image_max, value range of izipv bytes */
+  int32_t *pid;    /* np_image_max (if has_pid) */
+  int32_t *rhoc;   /* (1-ncb:nt+ncb)^3 x nnt^3 */
+  float *vfield;   /* 3 x (1-ncb:nt+ncb)^3 x nnt^3 */
+  int64_t *idx_b_l, *idx_b_r;                 /* (1-ncb:nt+ncb)^2 x nnt^3 */
+  int64_t *ppl0, *pplr, *pprl, *ppr0, *ppl, *ppr; /* nt^2 x nnt^3 */
+  int64_t nplocal, npglobal;
+  /* `save`d real ftemp of ext_pp_force (W/ext_pp_force.f90:20,123): never
+     reset, so it is a running max over all calls. */
+  double pp_ftemp;
+  /* statistics of the last call */
+  double overhead_tile, overhead_image;
+  int64_t itest1, npairs, lost;
+} ocube;
+
+/* status codes, one per reference `stop` condition */
+enum {
+  OC_OK = 0,
+  OC_ERR_TILE_OVERFLOW = 1,   /* W/update_particle.f90:88-94 */
+  OC_ERR_IMAGE_OVERFLOW = 2,  /* W/buffer_grid.f90:148-154 */
+  OC_ERR_PP_OVERFLOW = 3,     /* W/ext_pp_force.f90:41-45 */
+  OC_ERR_CHECKSUM = 4,        /* W/buffer_grid.f90:174-177 */
+  OC_ERR_COUNT = 5,           /* W/ext_pp_force.f90:156-160, update_particle.f90:245-250 */
+  OC_ERR_DISPLACEMENT = 6,    /* a particle left the double buffer of rhoce */
+  OC_ERR_ALLOC = 11,
+  OC_ERR_INVALID = 12
+};
+
+int oc_create(ocube *c, int64_t nc, int64_t nnt, int64_t ncb, int64_t izipx,
+              int64_t izipv, int64_t np_nc, int64_t pp_on, int64_t pp_range,
+              float rsoft, float image_buffer, float tile_buffer, int64_t has_pid);
+void oc_destroy(ocube *c);
+
+/* disjoint-state transfer (W/checkpoint.f90:38-44 layout) */
+int oc_set_disjoint(ocube *c, int64_t nplocal, const int16_t *xp, const int16_t *vp,
+                    const int32_t *pid, const int32_t *rhoc_phys, const float *vfield_phys);
+int oc_get_disjoint(const ocube *c, int16_t *xp, int16_t *vp, int32_t *pid,
+                    int32_t *rhoc_phys, float *vfield_phys);
+
+void oc_spine_image(ocube *c);
+int oc_buffer_grid(ocube *c);             /* buffer_np, buffer_vc, redistribute_cdm */
+void oc_buffer_xvp(ocube *c, int dox, int dov);
+
+/* update_xp; returns status, fills stats[0..3] = std_vsim_c, std_vsim_res,
+   std_vsim (raw FP64 sums), sigma_vi_new */
+int oc_update_xp(ocube *c, float dt_old, float dt, float vz_max, float sigma_vi,
+                 double *stats, float *sigma_vi_new);
+
+/* fine CIC deposit of one tile into rho_f(nfe+2,nfe,nfe); returns testrhof */
+double oc_fine_deposit(const ocube *c, int64_t itx, int64_t ity, int64_t itz,
+                       float mass_p, float *rho_f);
+/* fine kick of one tile from force_f(3, nfb:nfe-nfb+1, ...) */
+void oc_fine_kick(ocube *c, int64_t itx, int64_t ity, int64_t itz, const float *force_f,
+                  float a_mid, float dt, float sigma_vi, float sigma_vi_new);
+/* coarse CIC deposit of all tiles into r3(nc,nc,nc); returns testrhoc */
+double oc_coarse_deposit(const ocube *c, float mass_p, float *r3);
+/* coarse kick from force_c(3,0:nc+1,0:nc+1,0:nc+1); vmax[3] out */
+void oc_coarse_kick(ocube *c, const float *force_c, float a_mid, float dt,
+                    float sigma_vi, float *vmax);
+/* ext_pp_force; returns status; f2_max_pp out */
+int oc_pp_force(ocube *c, float a_mid, float dt, float sigma_vi, float mass_p,
+                float *f2_max_pp);
+
+/* codec helpers exposed for the known-answer tests */
+double oc_decode_x(int64_t izipx, int32_t code);        /* offset in a cell */
+double oc_decode_v(int64_t izipv, int32_t code, float sigma);
+int32_t oc_encode_v(int64_t izipv, double v, float sigma);
+int32_t oc_encode_x_drift(int64_t izipx, int32_t code, double dx_cells);
+float oc_tanf_code(int64_t izipv, int32_t code);
+
+/* reference-format disjoint state from continuous positions/velocities:
+   the count sort of WU/initial_conditions.f90:684-786 (plain cell order) */
+int oc_state_from_xv(ocube *c, int64_t n, const double *x, const double *v,
+                     const int32_t *pid, float sigma_vi);
+ocube *oc_alloc(void);
+void oc_free(ocube *c);
+void oc_info(const ocube *c, int64_t *out16);
+void oc_set_npglobal(ocube *c, int64_t n);
+#endif

@@ -1,0 +1,149 @@
+"""GPU parity of each stage of the step against the oracle, on identical
+inputs (tests/helpers.zeldovich_state).  Integer results (cells, counts,
+order, codes) must be bit-exact; deposits too (the GPU gathers in the
+reference's own summation order); forces within 1e-5 relative L2 in FP32."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import zeldovich_state, rel_l2, global_force_to_tile, global_from_tiles
+
+pytestmark = pytest.mark.gpu
+
+CASES = [dict(nc=24, nnt=2), dict(nc=16, nnt=1), dict(nc=24, nnt=1, izipx=1, izipv=2),
+         dict(nc=16, nnt=2, izipx=2, izipv=1)]
+
+
+def _ids(c):
+    return "nc{nc}_nnt{nnt}_x{ix}v{iv}".format(ix=c.get("izipx", 2), iv=c.get("izipv", 2), **c)
+
+
+def _pair(case, pp_on=False, seed=42):
+    from cafproject_amd.cube import Cube
+    ix, iv = case.get("izipx", 2), case.get("izipv", 2)
+    st = zeldovich_state(case["nc"], case["nnt"], seed=seed, izipx=ix, izipv=iv)
+    oc = O.OracleCube(case["nc"], case["nnt"], izipx=ix, izipv=iv, pp_on=pp_on)
+    oc.set_state(st["xp"], st["vp"], st["pid"], st["rhoc"], st["vfield"])
+    oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+    g = Cube(case["nc"], case["nnt"], izipx=ix, izipv=iv, pp_on=pp_on)
+    g.set_state(st["xp"], st["vp"], st["pid"], st["rhoc"], st["vfield"])
+    return st, oc, g
+
+
+def _same_state(a, b):
+    for k in ("rhoc", "xp", "vp", "pid", "vfield"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_set_get_roundtrip(case):
+    st, oc, g = _pair(case)
+    _same_state(g.get_state(), oc.get_state())
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+@pytest.mark.parametrize("dt", [1.0, 25.0, 70.0])
+def test_update_x_bit_exact(case, dt):
+    st, oc, g = _pair(case)
+    vz = float(np.abs(st["vfield"][:, 2]).max() * 1.5 + 0.02)
+    s_o, stats = oc.update_x(dt, dt, vz, st["sigma_vi"])
+    s_g = g.update_x(dt, dt, vz, st["sigma_vi"])
+    _same_state(g.get_state(), oc.get_state())
+    assert s_g["nplocal"] == oc.nplocal
+    assert abs(s_g["sum_r"] / stats[1] - 1) < 1e-12
+    assert abs(s_g["sigma_vi_new"] / s_o - 1) < 1e-6
+
+
+@pytest.mark.parametrize("case", CASES[:2], ids=_ids)
+def test_two_drifts_bit_exact(case):
+    st, oc, g = _pair(case)
+    vz = float(np.abs(st["vfield"][:, 2]).max() * 1.5 + 0.02)
+    sig = st["sigma_vi"]
+    for dt in (40.0, 55.0):
+        s_o, _ = oc.update_x(dt, dt, vz, sig)
+        g.update_x(dt, dt, vz, sig)
+        oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+        g.buffer_grid(); g.buffer_x(); g.buffer_v()
+        sig = s_o
+    _same_state(g.get_state(), oc.get_state())
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_fine_and_coarse_deposit_bit_exact(case):
+    st, oc, g = _pair(case)
+    m = st["mass_p"]
+    tf = g.fine_deposit(m)
+    rho_g = g.get_field("rho_f")
+    nc, nnt = case["nc"], case["nnt"]
+    nfb, nfe, nft = oc.nfb, oc.nfe, oc.nft
+    tiles = {}
+    s_o = 0.0
+    for itz in range(1, nnt + 1):
+        for ity in range(1, nnt + 1):
+            for itx in range(1, nnt + 1):
+                rho, s = oc.fine_deposit(itx, ity, itz, m)
+                s_o += s
+                tiles[(itx, ity, itz)] = dict(rho=rho[nfb - 1:nfb + nft + 1, nfb - 1:nfb + nft + 1,
+                                                      nfb - 1:nfb + nft + 1])
+    rho_o = global_from_tiles(tiles, nc, nnt, key="rho")
+    assert np.array_equal(rho_g, rho_o)
+    assert abs(tf / s_o - 1) < 1e-12
+    tc = g.coarse_deposit(m)
+    r3_o, sc = oc.coarse_deposit(m)
+    assert np.array_equal(g.get_field("rho_c"), r3_o)
+    assert abs(tc / sc - 1) < 1e-12
+
+
+@pytest.mark.parametrize("case", CASES[:2], ids=_ids)
+def test_forces_within_1e5(case):
+    st, oc, g = _pair(case)
+    m = st["mass_p"]
+    oc.init_kernels()
+    g.init_kernels()
+    pm = oc.particle_mesh(0.02, 0.0, st["sigma_vi"], st["sigma_vi"], m, keep_fields=True)
+    g.fine_deposit(m)
+    f2 = g.fine_force()
+    fo = global_from_tiles(pm["tiles"], case["nc"], case["nnt"])
+    fg = g.get_field("force_f")
+    assert rel_l2(fg, fo) < 1e-5, rel_l2(fg, fo)
+    assert abs(f2 / pm["f2_max_fine"] - 1) < 1e-4
+    g.coarse_deposit(m)
+    f2c = g.coarse_force()
+    fc_o = pm["force_c"][1:-1, 1:-1, 1:-1]
+    assert rel_l2(g.get_field("force_c"), fc_o) < 1e-5
+    assert abs(f2c / pm["f2_max_coarse"] - 1) < 1e-4
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_kicks_bit_exact_given_forces(case):
+    st, oc, g = _pair(case)
+    nc, nnt = case["nc"], case["nnt"]
+    rng = np.random.default_rng(7)
+    ff = rng.standard_normal((4 * nc,) * 3 + (3,)).astype(np.float32) * 3
+    g.set_field("force_f", ff)
+    a_mid, dt, s0, s1 = 0.03, 2.0, st["sigma_vi"], st["sigma_vi"] * 1.1
+    g.fine_kick(a_mid, dt, s0, s1)
+    for itz in range(1, nnt + 1):
+        for ity in range(1, nnt + 1):
+            for itx in range(1, nnt + 1):
+                tile = global_force_to_tile(ff, nc, nnt, itx, ity, itz)
+                O.lib().oc_fine_kick(oc._c, itx, ity, itz, O._p(tile), a_mid, dt, s0, s1)
+    _same_state(g.get_state(), oc.get_state())
+    fc = rng.standard_normal((nc,) * 3 + (3,)).astype(np.float32) * 50
+    g.set_field("force_c", fc)
+    vg = g.coarse_kick(a_mid, dt, s1)
+    halo = np.ascontiguousarray(np.pad(fc, ((1, 1), (1, 1), (1, 1), (0, 0)), mode="wrap"))
+    vo = np.zeros(3, np.float32)
+    O.lib().oc_coarse_kick(oc._c, O._p(halo), a_mid, dt, s1, O._p(vo))
+    _same_state(g.get_state(), oc.get_state())
+    assert np.array_equal(vg, vo)
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_pp_bit_exact(case):
+    st, oc, g = _pair(case, pp_on=True)
+    m = st["mass_p"]
+    f_o = oc.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
+    f_g = g.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
+    _same_state(g.get_state(), oc.get_state())
+    assert f_g == f_o
