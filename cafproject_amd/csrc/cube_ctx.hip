@@ -35,6 +35,8 @@ static void free_all(cube_ctx *c) {
     if (p) (void)hipFree(p);
   cube_fft_destroy(c->plan_f);
   cube_fft_destroy(c->plan_c);
+  if (c->tev[0]) (void)hipEventDestroy(c->tev[0]);
+  if (c->tev[1]) (void)hipEventDestroy(c->tev[1]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -195,6 +197,30 @@ extern "C" int cube_sync(cube_ctx *c) {
 
 extern "C" double cube_host_atan(double x) { return cube_atan(x); }
 
+extern "C" int cube_profile(cube_ctx *c, int on) {
+  if (!c) return CUBE_ERR_INVALID;
+  CUBE_CHECK(hipSetDevice(c->device));
+  if (on && !c->tev[0]) {
+    CUBE_CHECK(hipEventCreate(&c->tev[0]));
+    CUBE_CHECK(hipEventCreate(&c->tev[1]));
+  }
+  c->profiling = on ? 1 : 0;
+  return CUBE_OK;
+}
+
+extern "C" int cube_timers(cube_ctx *c, double *ms, int64_t *calls, int reset) {
+  if (!c) return CUBE_ERR_INVALID;
+  for (int i = 0; i < CUBE_NTIMERS; ++i) {
+    if (ms) ms[i] = c->tms[i];
+    if (calls) calls[i] = c->tcalls[i];
+    if (reset) {
+      c->tms[i] = 0;
+      c->tcalls[i] = 0;
+    }
+  }
+  return CUBE_OK;
+}
+
 /* ---- state transfer ---------------------------------------------------- */
 extern "C" int cube_set_state(cube_ctx *c, int64_t nplocal, int64_t npglobal, const void *xp, const void *vp,
                               const int32_t *pid, const int32_t *rhoc, const float *vfield) {
@@ -269,7 +295,9 @@ extern "C" int cube_buffer_grid(cube_ctx *c) {
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
   int64_t *sums = (int64_t *)malloc(c->ntiles * sizeof(int64_t));
+  tbeg(c);
   int st = cube_tile_boxsum(c, c->rhoc, (int)c->ncb, sums);
+  tend(c, CUBE_T_CHECKS);
   int64_t tot = 0;
   for (int64_t t = 0; t < c->ntiles; ++t) tot += sums[t];
   free(sums);
@@ -292,14 +320,18 @@ extern "C" int cube_pp_force(cube_ctx *c, float a_mid, float dt, float sigma_vi,
   CUBE_CHECK(hipSetDevice(c->device));
   /* W/ext_pp_force.f90:41-45: particles of coarse cells 0..nt+1 of a tile */
   int64_t *sums = (int64_t *)malloc(c->ntiles * sizeof(int64_t));
+  tbeg(c);
   int st = cube_tile_boxsum(c, c->rhoc, 1, sums);
+  tend(c, CUBE_T_CHECKS);
   int64_t mx = 0;
   for (int64_t t = 0; t < c->ntiles; ++t) mx = sums[t] > mx ? sums[t] : mx;
   free(sums);
   if (st) return st;
   if (mx > c->np_pp_max) return CUBE_ERR_PP_OVERFLOW;
   int64_t itest1 = 0;
+  tbeg(c);
   st = cube_pp(c, a_mid, dt, sigma_vi, mass_p, f2_max_pp, &itest1);
+  tend(c, CUBE_T_PP);
   if (st) return st;
   if (itest1 != c->nplocal) return CUBE_ERR_COUNT;
   return CUBE_OK;
@@ -309,41 +341,59 @@ extern "C" int cube_pm_fine_deposit(cube_ctx *c, float mass_p, double *testrhof)
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
-  return cube_fine_deposit(c, mass_p, testrhof);
+  tbeg(c);
+  const int st = cube_fine_deposit(c, mass_p, testrhof);
+  tend(c, CUBE_T_FINE_DEPOSIT);
+  return st;
 }
 extern "C" int cube_pm_fine_force(cube_ctx *c, float *f2) {
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_kernels) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
+  tbeg(c);
   int st = cube_fft_r2c_3d(c, c->plan_f, c->rho_f, c->spec_f);
   if (st) return st;
-  return cube_fft_green_c2r_3d(c, c->plan_f, c->spec_f, c->kern_f, 1, c->specd_f, c->force_f, (float)c->nf, f2);
+  tend(c, CUBE_T_FINE_FFT);
+  return cube_fft_green_c2r_3d(c, c->plan_f, c->spec_f, c->kern_f, 1, c->specd_f, c->force_f, (float)c->nf, f2,
+                               CUBE_T_FINE_GREEN);
 }
 extern "C" int cube_pm_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new) {
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
-  return cube_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new);
+  tbeg(c);
+  const int st = cube_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new);
+  tend(c, CUBE_T_FINE_KICK);
+  return st;
 }
 extern "C" int cube_pm_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc) {
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
-  return cube_coarse_deposit(c, mass_p, testrhoc);
+  tbeg(c);
+  const int st = cube_coarse_deposit(c, mass_p, testrhoc);
+  tend(c, CUBE_T_COARSE_DEPOSIT);
+  return st;
 }
 extern "C" int cube_pm_coarse_force(cube_ctx *c, float *f2) {
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_kernels) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
+  tbeg(c);
   int st = cube_fft_r2c_3d(c, c->plan_c, c->rho_c, c->spec_c);
   if (st) return st;
-  return cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc, f2);
+  st = cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc, f2, -1);
+  tend(c, CUBE_T_COARSE_FORCE);
+  return st;
 }
 extern "C" int cube_pm_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax) {
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
-  return cube_coarse_kick(c, a_mid, dt, sigma_vi, vmax);
+  tbeg(c);
+  const int st = cube_coarse_kick(c, a_mid, dt, sigma_vi, vmax);
+  tend(c, CUBE_T_COARSE_KICK);
+  return st;
 }
 
 /* particle_mesh, W/pm.f90:2-415 */

@@ -536,7 +536,9 @@ int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi
   /* tile overflow check (W/update_particle.f90:86-94): the copies in each
      tile's extended region must fit np_tile_max */
   int64_t *sums = (int64_t *)malloc(c->ntiles * sizeof(int64_t));
+  tbeg(c);
   int st = cube_tile_boxsum(c, c->rhoc, (int)c->ncb, sums);
+  tend(c, CUBE_T_CHECKS);
   double ov = 0;
   int64_t mx = 0;
   for (int64_t t = 0; t < c->ntiles; ++t) {
@@ -547,6 +549,7 @@ int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi
   free(sums);
   if (st) return st;
   if (mx > c->np_tile_max) return CUBE_ERR_TILE_OVERFLOW;
+  tbeg(c);
   if ((st = cube_vdec_table(c, c->vdec0, sigma_vi))) return st;
   DriftP P;
   P.g = geo_of(c);
@@ -563,6 +566,7 @@ int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi
   else if (c->p.izipv == 1) st = drift_impl<2, 1>(c, P, venc, &nxtra, hc);
   else st = drift_impl<2, 2>(c, P, venc, &nxtra, hc);
   if (st) return st;
+  tend(c, CUBE_T_DRIFT);
   /* the new state becomes current */
   void *tx = c->xp; c->xp = c->xp2; c->xp2 = tx;
   void *tv = c->vp; c->vp = c->vp2; c->vp2 = tv;
@@ -572,7 +576,9 @@ int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi
   int64_t *ts = c->cstart; c->cstart = c->cstart2; c->cstart2 = ts;
   /* std of the velocity field, FP64 sums (:171-224) */
   double s3[3];
+  tbeg(c);
   if ((st = cube_vstats_run(c, sigma_vi, s3))) return st;
+  tend(c, CUBE_T_VSTATS);
   const double npg = (double)c->npglobal;
   cube_vstats v;
   memset(&v, 0, sizeof(v));

@@ -551,7 +551,7 @@ int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out) {
 /* spec: output of cube_fft_r2c_3d (x and y transformed).  Produces the
    three force components (3 floats per cell) and max |F|^2. */
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
-                          float2 *work3, float *force3, float div, float *f2max) {
+                          float2 *work3, float *force3, float div, float *f2max, int t0) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, (unsigned *)c->ured, 1);
   CUBE_LAUNCH_CHECK();
   const int64_t nh = p->n / 2 + 1, plane = nh * (int64_t)p->n * p->n;
@@ -559,10 +559,14 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
 #define CASE(N)                                                                   \
   case N: {                                                                       \
     int st;                                                                       \
+    if (t0 >= 0) tbeg(c);                                                         \
     if ((st = Launch<N>::green(c, p, spec, kern, kern_octant, work3))) return st; \
+    if (t0 >= 0) { tend(c, t0); tbeg(c); }                                        \
     for (int d = 0; d < 3; ++d)                                                   \
       if ((st = Launch<N>::c2c_y<+1>(c, p, work3 + d * plane))) return st;        \
+    if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                    \
     if ((st = Launch<N>::c2r3(c, p, work3, force3, div, (unsigned *)c->ured))) return st; \
+    if (t0 >= 0) tend(c, t0 + 2);                                                 \
     break;                                                                        \
   }
     FFT_SIZES(CASE)

@@ -84,7 +84,26 @@ struct cube_ctx {
   /* host-side persistent state */
   float pp_ftemp;           /* running max of W/ext_pp_force.f90:123 */
   FFTPlan *plan_f, *plan_c;
+  /* stage timers */
+  int profiling;
+  hipEvent_t tev[2];
+  double tms[CUBE_NTIMERS];
+  int64_t tcalls[CUBE_NTIMERS];
 };
+
+/* stage timing on the context stream (no-ops unless profiling) */
+static inline void tbeg(cube_ctx *c) {
+  if (c->profiling) (void)hipEventRecord(c->tev[0], c->stream);
+}
+static inline void tend(cube_ctx *c, int id) {
+  if (!c->profiling) return;
+  (void)hipEventRecord(c->tev[1], c->stream);
+  (void)hipEventSynchronize(c->tev[1]);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->tev[0], c->tev[1]);
+  c->tms[id] += ms;
+  c->tcalls[id] += 1;
+}
 
 /* --- common device helpers --- */
 struct Geo {
@@ -124,7 +143,7 @@ int cube_fft_create(cube_ctx *c, int n, FFTPlan **out);
 void cube_fft_destroy(FFTPlan *p);
 int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
-                          float2 *work3, float *force3, float scale_div, float *f2max);
+                          float2 *work3, float *force3, float scale_div, float *f2max, int timer0);
 int cube_fft_forward_full(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out);
 

@@ -423,7 +423,10 @@ __global__ void k_lrckcorr(const float2 *spec_corr, const float2 *spec_plain, in
     const float kc = spec_corr[q].y;
     const float cu = spec_plain[q].y;
     float out = kc;
-    if (!(kr > 8.0f || ki[d] == 0)) {
+    /* for nc < 24 the |k|<=8 sphere reaches the Nyquist plane of axis d,
+       where Im FFT of the odd kernel is exactly zero and the reference
+       divides by it (NaN); those modes keep the uncorrected value */
+    if (!(kr > 8.0f || ki[d] == 0 || 2 * ki[d] == -nc)) {
       const float ssum = (ks[0] * ks[0] + ks[1] * ks[1]) + ks[2] * ks[2];
       out = ((((kc * 0.25f) * CUBE_PI_F) * ks[d]) / ssum) / cu;
     }

@@ -72,7 +72,10 @@ EXPORTS = ["cube_ctx_create", "cube_ctx_destroy", "cube_strerror", "cube_ctx_inf
            "cube_set_state", "cube_get_state", "cube_update_x", "cube_buffer_grid", "cube_buffer_x",
            "cube_pp_force", "cube_particle_mesh", "cube_buffer_v", "cube_pm_fine_deposit",
            "cube_pm_fine_force", "cube_pm_fine_kick", "cube_pm_coarse_deposit", "cube_pm_coarse_force",
-           "cube_pm_coarse_kick", "cube_get_field", "cube_set_field", "cube_sync", "cube_host_atan"]
+           "cube_pm_coarse_kick", "cube_get_field", "cube_set_field", "cube_sync", "cube_host_atan",
+           "cube_profile", "cube_timers"]
+TIMERS = ["drift", "vstats", "pp", "fine_deposit", "fine_fft", "fine_green", "fine_invy", "fine_c2r",
+          "fine_kick", "coarse_deposit", "coarse_force", "coarse_kick", "checks"]
 
 
 def lib():
@@ -107,6 +110,8 @@ def lib():
         L.cube_set_field.argtypes = [P, i32, P, ctypes.c_size_t]
         L.cube_host_atan.argtypes = [ctypes.c_double]
         L.cube_host_atan.restype = ctypes.c_double
+        L.cube_profile.argtypes = [P, i32]
+        L.cube_timers.argtypes = [P, P, P, i32]
         _LIB = L
     return _LIB
 
@@ -286,6 +291,15 @@ class Cube:
         if a.shape != self.field_shape(name):
             raise ValueError(f"{name}: shape {a.shape} != {self.field_shape(name)}")
         self._check(lib().cube_set_field(self._h, FIELD[name], _ptr(a), a.nbytes), "set_field")
+
+    def profile(self, on=True):
+        self._check(lib().cube_profile(self._h, int(on)), "profile")
+
+    def timers(self, reset=False):
+        ms = np.zeros(len(TIMERS), np.float64)
+        calls = np.zeros(len(TIMERS), np.int64)
+        self._check(lib().cube_timers(self._h, _ptr(ms), _ptr(calls), int(reset)), "timers")
+        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(TIMERS)}
 
     def sync(self):
         self._check(lib().cube_sync(self._h), "sync")

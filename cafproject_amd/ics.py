@@ -96,9 +96,13 @@ class Cosmology:
 # --------------------------------------------------------------------------
 # Zel'dovich particles
 # --------------------------------------------------------------------------
-def zeldovich(nc, box, z_i, seed, cosmo=None, np_nc=2, ncell=4, dtype=np.float32):
+def zeldovich(nc, box, z_i, seed, cosmo=None, np_nc=2, ncell=4, dtype=np.float32, workers=None):
     """Particle positions (global coarse-cell units, periodic in [0,nc)) and
     velocities (fine cells per unit time) on the np_nc-per-cell lattice."""
+    import os
+
+    import scipy.fft as sfft
+    workers = workers or os.cpu_count()
     if cosmo is None:
         cosmo = Cosmology()
     a = 1.0 / (1.0 + z_i)
@@ -107,38 +111,47 @@ def zeldovich(nc, box, z_i, seed, cosmo=None, np_nc=2, ncell=4, dtype=np.float32
     cell = box / ng                     # Mpc/h per lattice cell
     rng = np.random.default_rng(seed)
     w = rng.standard_normal((ng, ng, ng), dtype=np.float32)
-    wk = np.fft.rfftn(w).astype(np.complex64)
+    wk = sfft.rfftn(w, workers=workers)
     del w
     kf = 2 * np.pi * np.fft.fftfreq(ng)                 # radians per lattice cell
     kx = 2 * np.pi * np.fft.rfftfreq(ng)
-    kphys_y = np.abs(kf) / cell
-    kphys_x = np.abs(kx) / cell
-    # finite-difference k~^2 on the lattice, in lattice units (WU/initial_conditions.f90:440-446)
+    # P(k) tabulated once on a log grid and interpolated (k in h/Mpc)
+    kmin = 2 * np.pi / box * 0.5
+    kmax = np.sqrt(3.0) * np.pi / cell * 1.01
+    lk = np.linspace(np.log(kmin), np.log(kmax), 4096)
+    lpk = np.log(np.maximum(cosmo.pk(np.exp(lk), a), 1e-300))
     sy = (2 * np.sin(kf / 2)) ** 2
     sx = (2 * np.sin(kx / 2)) ** 2
+    ky2 = (kf / cell) ** 2
+    kx2 = (kx / cell) ** 2
     for iz in range(ng):
         kz = kf[iz]
-        kk = np.sqrt(kphys_y[:, None] ** 2 + kphys_x[None, :] ** 2 + (abs(kz) / cell) ** 2)
-        amp = np.sqrt(cosmo.pk(kk, a) / cell ** 3)
+        kk = np.sqrt(ky2[:, None] + kx2[None, :] + (kz / cell) ** 2)
+        amp = np.exp(0.5 * np.interp(np.log(np.maximum(kk, kmin)), lk, lpk)) / cell ** 1.5
         k2 = sy[:, None] + sx[None, :] + (2 * np.sin(kz / 2)) ** 2
         k2[k2 == 0] = 1.0
-        # phi = -4 pi delta / k~^2 in lattice units
-        wk[iz] = (wk[iz] * amp * (-4 * np.pi / k2)).astype(np.complex64)
+        # phi = -4 pi delta / k~^2 in lattice units (WU/initial_conditions.f90:433-447)
+        wk[iz] *= (amp * (-4 * np.pi / k2)).astype(np.float32)
     wk[0, 0, 0] = 0
-    phi = np.fft.irfftn(wk, s=(ng, ng, ng), axes=(0, 1, 2)).astype(np.float32)
+    phi = sfft.irfftn(wk, s=(ng, ng, ng), axes=(0, 1, 2), workers=workers).astype(np.float32)
     del wk
-    # displacement in lattice units: -(phi(i+1)-phi(i-1))/(8 pi) * 2 -> -grad phi/(4 pi)
-    q = (np.arange(ng, dtype=np.float64) + 0.25) / np_nc   # 0.5 fine cell into the cell
+    # displacement: -(phi(i+1)-phi(i-1))/(8 pi) lattice cells (central difference)
+    q = ((np.arange(ng, dtype=np.float64) + 0.25) / np_nc).astype(np.float32)
     x = np.empty((ng, ng, ng, 3), dtype)
     v = np.empty((ng, ng, ng, 3), dtype)
-    vf = vfactor(a, cosmo.omega_m, cosmo.omega_l)
-    for axis, name in ((2, 0), (1, 1), (0, 2)):
-        g = (np.roll(phi, -1, axis=axis) - np.roll(phi, 1, axis=axis))
-        psi_lat = -g / (8 * np.pi)                         # lattice cells
-        psi_fine = psi_lat * dx_fine
-        coord = np.broadcast_to(q.reshape([-1 if ax == axis else 1 for ax in range(3)]), phi.shape)
-        x[..., name] = (coord + psi_fine / ncell) % nc
-        v[..., name] = psi_fine * vf
+    vf = np.float32(vfactor(a, cosmo.omega_m, cosmo.omega_l))
+    scale = np.float32(-dx_fine / (8 * np.pi))
+    for axis, comp in ((2, 0), (1, 1), (0, 2)):
+        g = np.roll(phi, -1, axis=axis)
+        g -= np.roll(phi, 1, axis=axis)
+        g *= scale                                          # fine cells
+        shape = [1, 1, 1]
+        shape[axis] = ng
+        xc = q.reshape(shape) + g / np.float32(ncell)
+        np.mod(xc, np.float32(nc), out=xc)
+        x[..., comp] = xc
+        v[..., comp] = g * vf
+        del g, xc
     return x.reshape(-1, 3), v.reshape(-1, 3), a
 
 
@@ -161,16 +174,17 @@ def encode_v(izipv, v, sigma):
 
 
 def cell_ids(x, nc, nnt):
-    """Disjoint-order cell id (tile, k, j, i) of global coarse positions."""
+    """Disjoint-order cell id (tile, k, j, i) of global coarse positions in
+    [0, nc): cell ceiling(x) with x == 0 in the last cell."""
     nt = nc // nnt
-    g = np.ceil(np.mod(x, nc)).astype(np.int64)
-    g[g < 1] += nc
-    g[g > nc] -= nc
+    g = np.ceil(x).astype(np.int32)
     g -= 1
-    t = (g // nt)
-    l = g % nt
+    g[g < 0] += nc
+    g[g >= nc] -= nc
+    t = g // nt
+    g -= t * nt
     tile = t[:, 0] + nnt * (t[:, 1] + nnt * t[:, 2])
-    return l[:, 0] + nt * (l[:, 1] + nt * (l[:, 2] + nt * tile))
+    return (g[:, 0] + nt * (g[:, 1] + nt * (g[:, 2] + nt * tile.astype(np.int64)))).astype(np.int64)
 
 
 def disjoint_state(x, v, nc, nnt, izipx=2, izipv=2, sigma_vi=None, pid=None):

@@ -154,6 +154,31 @@ enum {
 int cube_get_field(cube_ctx *ctx, int which, void *host, size_t bytes);
 int cube_set_field(cube_ctx *ctx, int which, const void *host, size_t bytes);
 
+/* Per-stage device time, measured with HIP events on the context's stream
+   (the counterpart of the reference's per-stage system_clock prints, e.g.
+   W/pm.f90:195-196, W/update_particle.f90:251-252).  Stage ids: */
+enum {
+  CUBE_T_DRIFT = 0,        /* update_x: destination, sort, re-quantise     */
+  CUBE_T_VSTATS = 1,       /* update_x: velocity statistics                */
+  CUBE_T_PP = 2,           /* ext_pp_force                                 */
+  CUBE_T_FINE_DEPOSIT = 3, /* fine CIC deposit                             */
+  CUBE_T_FINE_FFT = 4,     /* fine r2c x + c2c y                           */
+  CUBE_T_FINE_GREEN = 5,   /* fine z pass: fft, i*K_d, inverse fft (x3)   */
+  CUBE_T_FINE_INVY = 6,    /* fine inverse y passes (x3)                   */
+  CUBE_T_FINE_C2R = 7,     /* fine c2r x of the three components + max F^2 */
+  CUBE_T_FINE_KICK = 8,    /* fine force gather + kick                     */
+  CUBE_T_COARSE_DEPOSIT = 9,
+  CUBE_T_COARSE_FORCE = 10,
+  CUBE_T_COARSE_KICK = 11,
+  CUBE_T_CHECKS = 12,      /* overflow checks (tile/image/pp box sums)     */
+  CUBE_NTIMERS = 13
+};
+/* on=1 starts recording (adds a synchronisation per stage), on=0 stops. */
+int cube_profile(cube_ctx *ctx, int on);
+/* ms[CUBE_NTIMERS] accumulated milliseconds, calls[CUBE_NTIMERS] launches
+   counted; reset=1 clears the accumulators after reading. */
+int cube_timers(cube_ctx *ctx, double *ms, int64_t *calls, int reset);
+
 /* Wait for all work of the context; returns CUBE_ERR_HIP on a device error. */
 int cube_sync(cube_ctx *ctx);
 

@@ -21,6 +21,7 @@ import scipy.fft as sfft
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+FFT_WORKERS = os.cpu_count()  # threads of the scipy FFTs (the CPU baseline sets 1)
 PI_F = np.float32(4.0) * np.arctan(np.float32(1.0))  # pi=4*atan(1.), real(4)
 
 
@@ -151,19 +152,20 @@ def kernel_f(nfe, fk, pp_on=False, pp_range=1, dtype=np.float32):
 
 def _rfftn(a):
     if a.dtype == np.float32:
-        return sfft.rfftn(a, workers=os.cpu_count())
+        return sfft.rfftn(a, workers=FFT_WORKERS)
     return np.fft.rfftn(a)
 
 
 def _irfftn_unnorm(c, n):
     if c.dtype == np.complex64:
-        return sfft.irfftn(c, s=(n, n, n), norm="forward", workers=os.cpu_count())
+        return sfft.irfftn(c, s=(n, n, n), norm="forward", workers=FFT_WORKERS)
     return np.fft.irfftn(c, s=(n, n, n), norm="forward")
 
 
-def kernel_c(nc, ckt, ncell=4, dtype=np.float32):
+def kernel_c(nc, ckt, ncell=4, dtype=np.float32, lrck=True, table=True):
     """kern_c(nc/2+1,nc,nc,3) of W/kernel_c.f90 with LRCKCORR, single image,
-    returned as [d][kz][ky][kx]."""
+    returned as [d][kz][ky][kx].  lrck/table switch off the low-k correction
+    (the #else branch, :326-332) or the 4^3 table octants (for the tests)."""
     f = dtype
     idx = (np.arange(1, nc + 1) + nc // 2 - 1) % nc - nc // 2
     rx = (ncell * idx).astype(f)
@@ -175,6 +177,8 @@ def kernel_c(nc, ckt, ncell=4, dtype=np.float32):
                        np.where(r == 0, f(0), -Z / r3)]).astype(f)      # [d][i][j][k]
     plain = ck.copy()
     t = ckt.astype(f)
+    if not table:
+        t = plain[:, :4, :4, :4].copy()
     R = slice(3, 0, -1)   # Fortran 4:2:-1 -> 0-based 3,2,1
     H = slice(nc - 3, nc)  # Fortran nc-2:nc
     Q = slice(0, 4)
@@ -202,7 +206,7 @@ def kernel_c(nc, ckt, ncell=4, dtype=np.float32):
         cu = _rfftn(np.ascontiguousarray(plain[d].transpose(2, 1, 0))).imag.astype(f)
         with np.errstate(divide="ignore", invalid="ignore"):
             corr = ((((kc * f(0.25)) * pi) * sd) / ssum) / cu
-        kern[d] = np.where((kr > 8.0) | (kd == 0), kc, corr)
+        kern[d] = np.where((kr > 8.0) | (kd == 0), kc, corr) if lrck else kc
     return kern
 
 

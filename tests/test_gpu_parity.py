@@ -94,7 +94,10 @@ def test_fine_and_coarse_deposit_bit_exact(case):
     assert abs(tc / sc - 1) < 1e-12
 
 
-@pytest.mark.parametrize("case", CASES[:2], ids=_ids)
+FORCE_CASES = [dict(nc=24, nnt=2), dict(nc=24, nnt=1, izipx=1, izipv=2)]
+
+
+@pytest.mark.parametrize("case", FORCE_CASES, ids=_ids)
 def test_forces_within_1e5(case):
     st, oc, g = _pair(case)
     m = st["mass_p"]
@@ -147,3 +150,14 @@ def test_pp_bit_exact(case):
     f_g = g.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
     _same_state(g.get_state(), oc.get_state())
     assert f_g == f_o
+
+
+def test_coarse_force_small_box_is_finite():
+    """nc < 24 is outside the reference's range (W/parameters.f90:29): its
+    LRCKCORR divides by the zero Nyquist mode.  The library keeps those
+    modes uncorrected, so the coarse force stays finite."""
+    st, oc, g = _pair(dict(nc=16, nnt=1))
+    g.init_kernels()
+    g.coarse_deposit(st["mass_p"])
+    f2 = g.coarse_force()
+    assert np.isfinite(f2) and np.all(np.isfinite(g.get_field("force_c")))
