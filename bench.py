@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-updates/s of the CUBE PM+PP step on one MI355X.
+
+Workload (the 512^3 configuration): 512^3 particles in a 400 Mpc/h periodic
+box, nc=256 coarse cells (1024^3 fine mesh), 2^3 tiles (ncb=6 buffers; the
+fine mesh is one periodic 1024^3 mesh on the GPU), int8 positions, int16
+velocities, PP on (pp_range=1, rsoft=0.3), Zel'dovich ICs at z=49 from
+seeded white noise (seed 42) and an Eisenstein-Hu no-wiggle LCDM spectrum
+(Om=0.31, Ob=0.049, h=0.68, ns=0.96, sigma8=0.81).
+
+One step is the reference's step loop body (CUBEnu/work/main/main.f90:102-118):
+timestep (host) -> update_x -> buffer_grid -> buffer_x -> ext_pp_force ->
+particle_mesh -> buffer_v.  Inputs are resident on the GPU before timing.
+
+Prints ONE JSON line.  `roofline` is for the dominant kernel, from HIP-event
+timings on the library's stream; `step_roofline` is the whole step against
+the algorithmic bytes B_step = N_p*beta_p + N_f*54 + N_c*86; `cpu_baseline`
+times the oracle (the C restatement of the reference path, with scipy FFTs)
+on one host core on a 128^3-particle sample of the same step.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s at 512^3 (PM+PP step); batched adjoint grads/s vs plain"
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_state_gpu(g, box, z_i, seed):
+    """Zel'dovich ICs generated on the GPU straight into the context."""
+    t = time.time()
+    a, sigma = g.make_ics(box, z_i, seed)
+    log(f"[bench] GPU Zel'dovich ICs: {g.nplocal} particles in {time.time() - t:.1f}s, sigma_vi={sigma:.5g}")
+    st = g.get_state()
+    nc = g.nc
+    # IC velocity constraint and vz_max (WU/initial_conditions.f90:631-641),
+    # from the decoded cell velocities plus residuals bounded by the codec
+    vf = st["vfield"]
+    vz = float(np.abs(vf[:, 2]).max()) + 3 * sigma * 2.5
+    vmax = float(np.abs(vf).max()) + 3 * sigma * 2.5
+    return dict(a=a, sigma_vi=sigma, np=st["nplocal"], vz_max=vz,
+                dt_vmax=float(np.float32(0.9) * np.float32(20) / np.float32(vmax)))
+
+
+class Step:
+    """The step loop body of W/main.f90:102-118 on the GPU context."""
+
+    def __init__(self, cube, st, mass_p, omega_m=0.31):
+        from cafproject_amd.timestep import Clock
+        self.g = cube
+        self.mass = mass_p
+        self.clock = Clock(st["a"], [0.0], omega_m, 1 - omega_m)
+        self.sigma = st["sigma_vi"]
+        self.vz_max = st["vz_max"]
+        self.dtc = dict(dt_fine=1000.0, dt_coarse=1000.0, dt_pp=1000.0, dt_vmax=st["dt_vmax"])
+
+    def __call__(self):
+        g, c = self.g, self.clock
+        c.timestep(**self.dtc)
+        vs = g.update_x(float(c.dt_old), float(c.dt), self.vz_max, self.sigma)
+        g.buffer_grid()
+        g.buffer_x()
+        a_mid, dt = float(c.a_mid), float(c.dt)
+        GG = np.float32(1.0) / np.float32(6.0) / np.float32(np.pi)
+        if g.pp_on:
+            f2pp = g.ext_pp_force(a_mid, dt, self.sigma, self.mass)
+            self.dtc["dt_pp"] = float(np.float32(1.0) / (np.sqrt(np.float32(f2pp)) * np.float32(a_mid) * GG))
+        pm = g.particle_mesh(a_mid, dt, self.sigma, vs["sigma_vi_new"], self.mass)
+        self.sigma = vs["sigma_vi_new"]
+        g.buffer_v()
+        self.dtc.update(dt_fine=pm["dt_fine"], dt_coarse=pm["dt_coarse"], dt_vmax=pm["dt_vmax"])
+        self.vz_max = pm["vz_max"]
+        return vs, pm
+
+
+def stage_bytes(np_, nf3, nc3, bx, bv):
+    """Algorithmic bytes per launch of each stage (the B_step split: every
+    stage reads its inputs once and writes its outputs once)."""
+    return {
+        "drift": np_ * (2 * bx + 2 * bv) + nc3 * 32,
+        "pp": np_ * (bx + 2 * bv),
+        "fine_deposit": np_ * bx + nf3 * 4,
+        "fine_fft": nf3 * 8,
+        "fine_inverse": nf3 * 30,
+        "fine_kick": np_ * (bx + 2 * bv) + nf3 * 12,
+        "coarse_deposit": np_ * bx + nc3 * 4,
+        "coarse_force": nc3 * 38,
+        "coarse_kick": np_ * (bx + 2 * bv) + nc3 * 12,
+    }
+
+
+MAIN_KERNEL = {
+    "drift": "k_drift_dest/k_place/k_sort_small/k_drift_final",
+    "pp": "k_pp", "fine_deposit": "k_fine_deposit", "fine_kick": "k_kick<fine>",
+    "coarse_deposit": "k_coarse_deposit", "coarse_kick": "k_kick<coarse>",
+    "fine_fft": "k_r2c_x+k_c2c_s", "fine_inverse": "k_green_z+k_c2c_s+k_c2r_x3",
+    "coarse_force": "coarse fft kernels",
+}
+
+
+def cpu_baseline(seconds_cap=60.0):
+    """Oracle step on a 128^3-particle sample (nc=64, 2^3 tiles, x1v2, PP on),
+    one host thread (scipy FFT workers=1)."""
+    from oracle import oracle as O
+    from cafproject_amd import ics
+    O.FFT_WORKERS = 1
+    nc, nnt = 64, 2
+    x, v, a = ics.zeldovich(nc, 100.0, 49.0, seed=42)
+    st = ics.disjoint_state(x, v, nc, nnt, izipx=1, izipv=2)
+    n = x.shape[0]
+    oc = O.OracleCube(nc, nnt, izipx=1, izipv=2, pp_on=True, has_pid=False)
+    oc.set_state(st["xp"], st["vp"], None, st["rhoc"], st["vfield"])
+    oc.init_kernels()
+    oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+    m = float(np.float32((4 * nc) ** 3 / n))
+    sig = st["sigma_vi"]
+    vz = float(np.abs(v[:, 2]).max())
+    t0 = time.perf_counter()
+    s2, _ = oc.update_x(0.0, 0.5, vz, sig)
+    oc.buffer_grid(); oc.buffer_x()
+    oc.ext_pp_force(a, 0.5, sig, m)
+    oc.particle_mesh(a, 0.5, sig, s2, m)
+    oc.buffer_v()
+    t = time.perf_counter() - t0
+    return dict(value=n / t, unit="particle-updates/s", cores=1, kind="port",
+                sample=f"one oracle step (update_x, buffers, PP, particle_mesh) of {n} particles "
+                       f"(nc=64, 2^3 tiles, x1v2, PP on, z=49 ICs) in {t:.1f} s on 1 host thread",
+                seconds=t)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nc", type=int, default=256)
+    ap.add_argument("--nnt", type=int, default=2)
+    ap.add_argument("--box", type=float, default=400.0)
+    ap.add_argument("--no-pp", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU (the whole box on one device); "
+              f"--gpus {args.gpus} is not supported. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    from cafproject_amd.cube import Cube
+    nc, nnt = args.nc, args.nnt
+    izipx, izipv = 1, 2
+    pp_on = not args.no_pp
+    t = time.time()
+    g = Cube(nc, nnt, izipx=izipx, izipv=izipv, has_pid=False, pp_on=pp_on, pp_range=1, rsoft=0.3)
+    g.init_kernels()
+    log(f"[bench] context + Green's functions in {time.time() - t:.1f}s")
+    st = make_state_gpu(g, args.box, 49.0, 42)
+    np_ = st["np"]
+    mass = float(np.float32((4 * nc) ** 3 / np_))
+    step = Step(g, st, mass)
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        step()
+        log(f"[bench] warmup step {i} {1e3 * (time.perf_counter() - t):.1f} ms a={float(step.clock.a):.5f}")
+    g.sync()
+    g.profile(True)
+    g.timers(reset=True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    tm = g.timers()
+    g.profile(False)
+    ms_step = 1e3 * elapsed / args.steps
+    value = np_ * args.steps / elapsed
+    bx, bv = 3 * izipx, 3 * izipv
+    nf3, nc3 = (4 * nc) ** 3, nc ** 3
+    beta = 7 * bx + 8 * bv if pp_on else 6 * bx + 6 * bv
+    B_step = np_ * beta + nf3 * 54 + nc3 * 86
+    sb = stage_bytes(np_, nf3, nc3, bx, bv)
+    if not pp_on:
+        sb.pop("pp")
+    stage_ms = {k: v[0] / max(v[1], 1) for k, v in tm.items()}
+    stage_ms["fine_inverse"] = sum(stage_ms[k] for k in ("fine_green", "fine_invy", "fine_c2r"))
+    per_launch = {k: dict(ms=round(stage_ms[k], 4), bytes=int(b),
+                          gbps=round(b / (stage_ms[k] * 1e-3) / 1e9, 1) if stage_ms[k] > 0 else None)
+                  for k, b in sb.items()}
+    single = [k for k in ("fine_deposit", "pp", "fine_kick", "coarse_deposit", "coarse_kick") if k in per_launch]
+    dom = max(single, key=lambda k: per_launch[k]["ms"])
+    d = per_launch[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3)
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"CUBE PM+PP step, {round(np_ ** (1 / 3))}^3 particles, {args.box:g} Mpc/h, "
+                               f"nc={nc} ({4 * nc}^3 fine mesh), {nnt}^3 tiles, int8 x / int16 v, "
+                               f"PP {'on' if pp_on else 'off'}, z=49 Zel'dovich EH no-wiggle, seed 42",
+                   "particles": np_, "nc": nc, "nnt": nnt, "nf": 4 * nc, "pp": pp_on,
+                   "izipx": izipx, "izipv": izipv},
+        "roofline": {"bound": "hbm", "kernel": MAIN_KERNEL[dom], "stage": dom,
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes_per_launch": d["bytes"], "avg_launch_ms": d["ms"]},
+        "step_roofline": {"B_step": B_step, "achieved_GBps": B_step / (ms_step * 1e-3) / 1e9,
+                          "frac": B_step / (ms_step * 1e-3) / HBM_PEAK},
+        "stages": per_launch,
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "a_final": float(step.clock.a),
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline must not hide the GPU number
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -476,6 +476,42 @@ int cube_vstats_run(cube_ctx *c, float sigma_vi, double *sums3) {
   return CUBE_OK;
 }
 
+/* ---------------- helpers shared with the IC sort ---------------- */
+__global__ void k_fill_zero_i32(int32_t *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+__global__ void k_place_keys(const int32_t *dest, const uint64_t *key, int64_t n, const int64_t *start, int32_t *fill,
+                             uint64_t *out) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t d = dest[p];
+    if (d < 0) continue;
+    const int32_t s = atomicAdd(fill + d, 1);
+    out[start[d] + s] = key[p];
+  }
+}
+/* sort every cell's key segment ascending (small ones in registers, big
+   ones with a workgroup bitonic sort) */
+extern "C" int cube_sort_small_segments(cube_ctx *c, const int32_t *cnt, const int64_t *start, uint64_t *keys) {
+  unsigned long long *ctr = (unsigned long long *)c->ured;
+  int st = fill_i64(c, (int64_t *)ctr + 3, 1, 0);
+  if (st) return st;
+  const unsigned nbc = (unsigned)((c->ncells + 255) / 256);
+  int32_t *biglist = (int32_t *)c->dest;
+  hipLaunchKernelGGL(k_sort_small, dim3(nbc), dim3(256), 0, c->stream, cnt, start, c->ncells, keys, biglist, ctr);
+  CUBE_LAUNCH_CHECK();
+  unsigned long long hc[8];
+  CUBE_CHECK(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
+  if (hc[3]) {
+    hipLaunchKernelGGL(k_sort_big, dim3((unsigned)hc[3]), dim3(1024), BIG_SORT * sizeof(uint64_t), c->stream, cnt,
+                       start, biglist, keys, ctr);
+    CUBE_LAUNCH_CHECK();
+    CUBE_CHECK(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
+    if (hc[2] & DERR_BIG) return CUBE_ERR_INVALID;
+  }
+  return CUBE_OK;
+}
+
 /* ---------------- driver ---------------- */
 template <int IX, int IV> static int drift_impl(cube_ctx *c, DriftP P, double venc, int64_t *nxtra,
                                                 unsigned long long *hc) {

@@ -155,5 +155,8 @@ int cube_green_init(cube_ctx *c, const float *wfxyzf, const float *wfxyzc);
 int cube_pp(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, float *f2_max_pp,
             int64_t *itest1);
 int cube_vstats_run(cube_ctx *c, float sigma_vi, double *sums3);
+__global__ void k_fill_zero_i32(int32_t *p, int64_t n);
+__global__ void k_place_keys(const int32_t *dest, const uint64_t *key, int64_t n, const int64_t *start, int32_t *fill,
+                             uint64_t *out);
 
 #endif

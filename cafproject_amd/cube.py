@@ -59,6 +59,12 @@ class PMOut(ctypes.Structure):
                 ("testrhof", ctypes.c_double), ("testrhoc", ctypes.c_double)]
 
 
+class ICParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("box", ctypes.c_double), ("vfactor", ctypes.c_double),
+                ("sigma_vi", ctypes.c_float), ("ntab", ctypes.c_int32), ("lnk0", ctypes.c_double),
+                ("dlnk", ctypes.c_double), ("lnp", ctypes.c_void_p)]
+
+
 def _struct_dict(s):
     out = {}
     for name, _ in s._fields_:
@@ -73,7 +79,7 @@ EXPORTS = ["cube_ctx_create", "cube_ctx_destroy", "cube_strerror", "cube_ctx_inf
            "cube_pp_force", "cube_particle_mesh", "cube_buffer_v", "cube_pm_fine_deposit",
            "cube_pm_fine_force", "cube_pm_fine_kick", "cube_pm_coarse_deposit", "cube_pm_coarse_force",
            "cube_pm_coarse_kick", "cube_get_field", "cube_set_field", "cube_sync", "cube_host_atan",
-           "cube_profile", "cube_timers"]
+           "cube_profile", "cube_timers", "cube_ic_zeldovich", "cube_set_state_xv"]
 TIMERS = ["drift", "vstats", "pp", "fine_deposit", "fine_fft", "fine_green", "fine_invy", "fine_c2r",
           "fine_kick", "coarse_deposit", "coarse_force", "coarse_kick", "checks"]
 
@@ -110,6 +116,8 @@ def lib():
         L.cube_set_field.argtypes = [P, i32, P, ctypes.c_size_t]
         L.cube_host_atan.argtypes = [ctypes.c_double]
         L.cube_host_atan.restype = ctypes.c_double
+        L.cube_ic_zeldovich.argtypes = [P, ctypes.POINTER(ICParams), P]
+        L.cube_set_state_xv.argtypes = [P, i64, P, P, f32, P]
         L.cube_profile.argtypes = [P, i32]
         L.cube_timers.argtypes = [P, P, P, i32]
         _LIB = L
@@ -202,6 +210,32 @@ class Cube:
         n = xp.shape[0]
         self._check(lib().cube_set_state(self._h, n, n if npglobal is None else npglobal, _ptr(xp),
                                          _ptr(vp), _ptr(pid), _ptr(rhoc), _ptr(vfield)), "set_state")
+
+    def set_state_xv(self, x, v, sigma_vi=0.0):
+        """Count-sort continuous particles (global coarse units) into the
+        disjoint state; returns the sigma_vi used."""
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float32)
+        s = ctypes.c_float()
+        self._check(lib().cube_set_state_xv(self._h, x.shape[0], _ptr(x), _ptr(v), sigma_vi, ctypes.byref(s)),
+                    "set_state_xv")
+        return s.value
+
+    def make_ics(self, box, z_i, seed, cosmo=None, sigma_vi=0.0, ntab=4096):
+        """Zel'dovich ICs on the GPU (cube_ic_zeldovich); returns (a, sigma_vi)."""
+        from cafproject_amd import ics
+        cosmo = cosmo or ics.Cosmology()
+        a = 1.0 / (1.0 + z_i)
+        ng = self.nc * self.p.np_nc
+        kmin = 2 * np.pi / box * 0.25
+        kmax = np.sqrt(3.0) * np.pi * ng / box * 1.05
+        lnk = np.linspace(np.log(kmin), np.log(kmax), ntab)
+        lnp = np.ascontiguousarray(np.log(np.maximum(cosmo.pk(np.exp(lnk), a), 1e-300)), np.float64)
+        ip = ICParams(seed, box, ics.vfactor(a, cosmo.omega_m, cosmo.omega_l), sigma_vi, ntab, float(lnk[0]),
+                      float(lnk[1] - lnk[0]), lnp.ctypes.data_as(ctypes.c_void_p))
+        s = ctypes.c_float()
+        self._check(lib().cube_ic_zeldovich(self._h, ctypes.byref(ip), ctypes.byref(s)), "make_ics")
+        return a, s.value
 
     def get_state(self):
         n = self.nplocal

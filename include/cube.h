@@ -142,6 +142,30 @@ int cube_pm_coarse_deposit(cube_ctx *ctx, float mass_p, double *testrhoc);
 int cube_pm_coarse_force(cube_ctx *ctx, float *f2_max_coarse);
 int cube_pm_coarse_kick(cube_ctx *ctx, float a_mid, float dt, float sigma_vi, float *vmax);
 
+/* Initial conditions (the IC generator CUBEnu/work/utilities/
+   initial_conditions.f90, on the GPU): seeded white noise on the particle
+   lattice (np_nc*nc per dim), delta_k = w_k sqrt(P(k)/cell^3), phi_k =
+   -4 pi delta_k/k~^2, Zel'dovich x and v from central differences of phi,
+   then the count sort into the disjoint state (xp, vp, pid, rhoc, vfield).
+   P(k) at the initial scale factor comes as a table of ln P on a uniform
+   ln k grid (k in h/Mpc).  sigma_vi <= 0 measures it from the residual
+   velocities; the value used is returned. */
+typedef struct {
+  uint64_t seed;
+  double box;        /* Mpc/h */
+  double vfactor;    /* velocity factor of the reference IC, vfactor(a) */
+  float sigma_vi;    /* <= 0: measure */
+  int32_t ntab;
+  double lnk0, dlnk; /* ln k of the first entry, spacing */
+  const double *lnp; /* ntab values of ln P(k) [(Mpc/h)^3] */
+} cube_ic_params;
+int cube_ic_zeldovich(cube_ctx *ctx, const cube_ic_params *ip, float *sigma_vi_out);
+/* Count sort of continuous particles into the disjoint state: x in global
+   coarse-cell units (n x 3 doubles), v in simulation units (n x 3 floats),
+   cell = ceiling(x), input order kept inside a cell (WU/:684-786). */
+int cube_set_state_xv(cube_ctx *ctx, int64_t n, const double *x, const float *v, float sigma_vi,
+                      float *sigma_vi_out);
+
 /* Mesh fields, for diagnostics and checks.  which: */
 enum {
   CUBE_FIELD_RHO_F = 0,   /* nf^3 float, fine density, (x,y,z) x fastest           */

@@ -950,8 +950,18 @@ int oc_state_from_xv(ocube *c, int64_t n, const double *x, const double *v,
   }
   start[0] = 0;
   for (int64_t q = 0; q < nphys; q++) start[q + 1] = start[q] + cnt[q];
-  for (int64_t q = 0; q < nphys; q++)
-    for (int d = 0; d < 3; d++) vf[3 * q + d] = (float)(vsum[3 * q + d] / (double)(cnt[q] ? cnt[q] : 1));
+  /* vfield: real(4) running sum in input order, then /merge(1,n,n==0)
+     (WU/initial_conditions.f90:723,732-734) */
+  {
+    float *acc = calloc(3 * nphys, sizeof(float));
+    for (int64_t p = 0; p < n; p++)
+      for (int d = 0; d < 3; d++)
+        acc[3 * cid[p] + d] = (float)((double)acc[3 * cid[p] + d] + v[3 * p + d]);
+    for (int64_t q = 0; q < nphys; q++)
+      for (int d = 0; d < 3; d++) vf[3 * q + d] = acc[3 * q + d] / (float)(cnt[q] ? cnt[q] : 1);
+    free(acc);
+  }
+  (void)vsum;
   int64_t *fill = calloc(nphys, sizeof(int64_t));
   for (int64_t p = 0; p < n; p++) {
     const int64_t q = cid[p], slot = start[q] + fill[q]++;
