@@ -57,7 +57,28 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+FORT = os.path.join(HERE, "fortran")
+FC = os.environ.get("CUBE_FC", "/opt/rocm/bin/amdflang")
+DRIVER = os.path.join(FORT, "cube_main")
+
+
+def build_fortran():
+    """The Fortran driver (fortran/cube_main.f90 over the ISO_C_BINDING
+    module fortran/cube_c.f90), linked against libcube_hip.so."""
+    mod = os.path.join(BUILD, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    tmp = DRIVER + ".tmp"
+    cmd = [FC, "-O2", "-module-dir", mod, os.path.join(FORT, "cube_c.f90"), os.path.join(FORT, "cube_main.f90"),
+           "-o", tmp, "-L" + HERE, "-lcube_hip", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -23,6 +23,7 @@
 #include "cube_internal.h"
 #include <string.h>
 #include <math.h>
+#include <utility>
 
 #define FFT_T 256
 
@@ -214,6 +215,260 @@ template <int N> struct Cfg {
   static constexpr int BC0 = 2048 / L;
   static constexpr int BC = BC0 < 1 ? 1 : (BC0 > 16 ? 16 : BC0);
 };
+
+/* ---------------- register-resident line FFTs ----------------
+ * For power-of-two N = R1*R2 (64..1024) a strided line FFT keeps the line
+ * in registers: a workgroup takes CB=16 adjacent columns (128-byte global
+ * segments) and R1 threads per column, thread t holding x[t + R1*m],
+ * m < R2.  Four-step: R2-point FFT in registers, twiddle by W_N^{t*q},
+ * one transpose through LDS, R1-point FFTs in registers; the result comes
+ * back in the same distribution (thread t holds X[t + R1*m]).  One LDS
+ * round trip per line instead of one per radix pass, and 16-column
+ * batches instead of 8 for the z lines.
+ */
+constexpr double ct_sin_q(double x) { /* Taylor series, x in [0, pi/2] */
+  double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ct_cos_q(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+/* exp(DIR * 2 pi i E / M) rounded to float, exact at quarter turns */
+template <int M, int E, int DIR> struct TwC {
+  static constexpr int e = ((E % M) + M) % M;
+  static constexpr int quad = (4 * e) / M;
+  static constexpr double x = 1.57079632679489661923 * (double)(4 * e - quad * M) / (double)M;
+  static constexpr double c = ct_cos_q(x), s = ct_sin_q(x);
+  static constexpr double cr = quad == 0 ? c : quad == 1 ? -s : quad == 2 ? -c : s;
+  static constexpr double ci = quad == 0 ? s : quad == 1 ? c : quad == 2 ? -s : -c;
+  static constexpr float re = (float)cr;
+  static constexpr float im = (float)(DIR * ci);
+};
+
+template <int M, int K, int DIR, int... Q>
+__device__ __forceinline__ void tw_apply(float2 *a, std::integer_sequence<int, Q...>) {
+  ((((K * Q) % M) != 0 ? (void)(a[Q] = cmul(a[Q], make_float2(TwC<M, K * Q, DIR>::re, TwC<M, K * Q, DIR>::im)))
+                       : (void)0),
+   ...);
+}
+/* butterfly J of one in-register Stockham pass (radix r, sub-size NS) */
+template <int R, int DIR, int r, int NS, int J>
+__device__ __forceinline__ void reg_bfly(const float2 *v, float2 *o) {
+  constexpr int NB = R / r, k = J % NS;
+  float2 a[r];
+#pragma unroll
+  for (int q = 0; q < r; ++q) a[q] = v[J + q * NB];
+  if constexpr (NS > 1) tw_apply<NS * r, k, DIR>(a, std::make_integer_sequence<int, r>{});
+  DFT<r, DIR>::run(a);
+  constexpr int base = (J / NS) * NS * r + k;
+#pragma unroll
+  for (int q = 0; q < r; ++q) o[base + q * NS] = a[q];
+}
+template <int R, int DIR, int r, int NS, int... J>
+__device__ __forceinline__ void reg_pass(const float2 *v, float2 *o, std::integer_sequence<int, J...>) {
+  (reg_bfly<R, DIR, r, NS, J>(v, o), ...);
+}
+/* in-register FFT of R points, natural order in and out */
+template <int R, int DIR, int P = 0>
+__device__ __forceinline__ void reg_fft(float2 *v) {
+  if constexpr (P < npass(R)) {
+    float2 o[R];
+    reg_pass<R, DIR, radix_at(R, P), ns_at(R, P)>(v, o, std::make_integer_sequence<int, R / radix_at(R, P)>{});
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = o[i];
+    reg_fft<R, DIR, P + 1>(v);
+  }
+}
+
+/* base + a byte offset known to fit 32 bits: lets the compiler keep the
+   uniform base in scalar registers and the per-lane offset in one VGPR */
+template <class T> __device__ __forceinline__ T *boff(T *base, uint32_t bytes) {
+  return reinterpret_cast<T *>(reinterpret_cast<char *>(const_cast<void *>(static_cast<const void *>(base))) + bytes);
+}
+/* a wave-uniform pointer pinned to scalar registers; `salt` is an opaque
+   zero that keeps separate uses (loads before an FFT, stores after it) from
+   being merged into one long-lived vector address */
+template <class T> __device__ __forceinline__ T *uni(T *p, int salt) {
+  const uint64_t u = reinterpret_cast<uint64_t>(p) + (uint64_t)salt;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ int opaque_zero() {
+  int z = 0;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  return z;
+}
+
+template <int N> struct RC {
+  static constexpr bool ok = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024;
+  static constexpr int R1 = N <= 128 ? 8 : (N <= 512 ? 16 : 32);
+  static constexpr int R2 = N / R1;
+  static constexpr int CB = 16;
+  static constexpr int NT = CB * R1;
+  /* LDS transpose buffer S[c][q][t]: q stride R1+1, c stride = 4 (mod 64)
+     so both the row writes and the column reads of a wave hit distinct banks */
+  static constexpr int SQ = R1 + 1;
+  static constexpr int SC0 = R2 * SQ;
+  static constexpr int SC = SC0 + ((4 - SC0) % 64 + 64) % 64;
+};
+
+/* line FFT of the CB columns of a workgroup; on entry v[m] = x[t + R1 m] of
+   column c, on exit v[m] = X[t + R1 m].  twl: W_N^e, e < N, in LDS. */
+template <int N, int DIR>
+__device__ __forceinline__ void blk_fft(float2 *v, int c, int t, float *S, const float2 *twl) {
+  using G = RC<N>;
+  constexpr int R1 = G::R1, R2 = G::R2, J = R2 / R1, SQ = G::SQ;
+  reg_fft<R2, DIR>(v);
+  /* twiddles W_N^{t q} = W_N^{8 t qh} W_N^{t ql}, q = 8 qh + ql: 7 + R2/8-1
+     table reads instead of R2-1 (fewer live registers, one extra rounding).
+     The opaque zero offset stops the reads of successive calls in one
+     kernel from being merged and held in registers between them. */
+  const int t0 = opaque_zero();
+  constexpr int QL = R2 < 8 ? R2 : 8, QH = R2 / QL;
+  float2 lo[QL];
+  lo[0] = make_float2(1.f, 0.f);
+#pragma unroll
+  for (int ql = 1; ql < QL; ++ql) {
+    lo[ql] = twl[t0 + t * ql];
+    if (DIR > 0) lo[ql].y = -lo[ql].y;
+  }
+#pragma unroll
+  for (int qh = 0; qh < QH; ++qh) {
+    float2 hi = make_float2(1.f, 0.f);
+    if (qh > 0) {
+      hi = twl[t0 + QL * t * qh];
+      if (DIR > 0) hi.y = -hi.y;
+    }
+#pragma unroll
+    for (int ql = 0; ql < QL; ++ql) {
+      const int q = qh * QL + ql;
+      if (q == 0) continue;
+      const float2 w = qh == 0 ? lo[ql] : (ql == 0 ? hi : cmul(hi, lo[ql]));
+      v[q] = cmul(v[q], w);
+    }
+  }
+  float *Sc = S + c * G::SC;
+  float2 a[R2];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < R2; ++q) Sc[q * SQ + t] = v[q].x;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+#pragma unroll
+    for (int p = 0; p < R1; ++p) a[j * R1 + p].x = Sc[(t + R1 * j) * SQ + p];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < R2; ++q) Sc[q * SQ + t] = v[q].y;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+#pragma unroll
+    for (int p = 0; p < R1; ++p) a[j * R1 + p].y = Sc[(t + R1 * j) * SQ + p];
+#pragma unroll
+  for (int j = 0; j < J; ++j) reg_fft<R1, DIR>(a + j * R1);
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+#pragma unroll
+    for (int p = 0; p < R1; ++p) v[j + J * p] = a[j * R1 + p];
+}
+
+/* in-place strided line FFTs, register version: element (col, n) of line
+   set (blockIdx.y, blockIdx.z) at a + z*cstride + y*ostride + n*stride + col.
+   Requires R1*stride*8 < 2^32 (true for every mesh up to 1024^3). */
+template <int N, int DIR>
+__global__ __launch_bounds__(RC<N>::NT) void k_c2c_r(float2 *__restrict__ a, const float2 *__restrict__ tw,
+                                                   int ncol, int64_t stride, int64_t ostride, int64_t cstride) {
+  using G = RC<N>;
+  constexpr int R1 = G::R1, R2 = G::R2;
+  __shared__ float S[G::CB * G::SC];
+  __shared__ float2 twl[N];
+  const int tid = threadIdx.x, c = tid % G::CB, t = tid / G::CB;
+  for (int e = tid; e < N; e += G::NT) twl[e] = tw[e];
+  const int col = blockIdx.x * G::CB + c;
+  const bool ok = col < ncol;
+  /* columns past the end compute on a copy of the last one and store nothing */
+  const uint32_t lo = (uint32_t)(((int64_t)t * stride + (ok ? col : ncol - 1)) * 8);
+  float2 *blk = a + (int64_t)blockIdx.z * cstride + (int64_t)blockIdx.y * ostride;
+  float2 v[R2];
+  const int s0 = opaque_zero();
+#pragma unroll
+  for (int m = 0; m < R2; ++m) v[m] = *boff(uni(blk + (int64_t)(R1 * m) * stride, s0), lo);
+  __syncthreads();
+  blk_fft<N, DIR>(v, c, t, S, twl);
+  if (ok) {
+    const int s1 = opaque_zero();
+#pragma unroll
+    for (int m = 0; m < R2; ++m) *boff(uni(blk + (int64_t)(R1 * m) * stride, s1), lo) = v[m];
+  }
+}
+
+/* Green's function pass along z, register version of k_green_z; OCT selects
+   the octant kernel storage (fine mesh) over the full one (coarse mesh) */
+template <int N, bool OCT>
+__global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict__ spec, float2 *__restrict__ outd,
+                                                     const float *__restrict__ kern,
+                                                     const float2 *__restrict__ tw) {
+  using G = RC<N>;
+  constexpr int NH = N / 2 + 1, H = N / 2, R1 = G::R1, R2 = G::R2, H1 = H + 1;
+  constexpr int64_t stride = (int64_t)N * NH, plane = (int64_t)NH * N * N;
+  __shared__ float S[G::CB * G::SC];
+  __shared__ float2 twl[N];
+  const int tid = threadIdx.x, c = tid % G::CB, t = tid / G::CB;
+  for (int e = tid; e < N; e += G::NT) twl[e] = tw[e];
+  const int kx0 = blockIdx.x * G::CB + c, y = blockIdx.y;
+  const bool ok = kx0 < NH;
+  const int kx = ok ? kx0 : NH - 1;
+  const uint32_t lo = (uint32_t)(((int64_t)t * stride + kx) * 8);
+  const float2 *src = spec + (int64_t)y * NH;
+  float2 v[R2];
+  const int s0 = opaque_zero();
+#pragma unroll
+  for (int m = 0; m < R2; ++m) v[m] = *boff(uni(src + (int64_t)(R1 * m) * stride, s0), lo);
+  __syncthreads();
+  blk_fft<N, -1>(v, c, t, S, twl);
+  const int ky = y <= H ? y : N - y;
+  const float sy = y <= H ? 1.f : -1.f;
+#pragma unroll 1
+  for (int d = 0; d < 3; ++d) {
+    const int z0 = opaque_zero();
+    float2 w[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      const int n = t + R1 * m;
+      float K;
+      if constexpr (OCT) {
+        const int kz = n <= H ? n : N - n;
+        const float *kb = uni(kern + ((int64_t)d * H1 * H1 + ky) * H1, z0);
+        K = *boff(kb, (uint32_t)((kz * H1 * H1 + kx) * 4));
+        if (d == 1) K *= sy;
+        if (d == 2 && n > H) K = -K;
+      } else {
+        const float *kb = uni(kern + ((int64_t)d * N * N + y) * NH, z0);
+        K = *boff(kb, (uint32_t)((n * N * NH + kx) * 4));
+      }
+      /* rho_f(::2)=-crho(2::2)*K, rho_f(2::2)=crho(::2)*K  (W/pm.f90:119-120) */
+      w[m] = make_float2((-v[m].y) * K, v[m].x * K);
+    }
+    blk_fft<N, +1>(w, c, t, S, twl);
+    if (ok) {
+      float2 *dst = outd + d * plane + (int64_t)y * NH;
+      const int s1 = opaque_zero();
+#pragma unroll
+      for (int m = 0; m < R2; ++m) *boff(uni(dst + (int64_t)(R1 * m) * stride, s1), lo) = w[m];
+    }
+  }
+}
 
 /* ---------------- kernels ---------------- */
 /* real rows (N floats) -> half spectra rows (NH complex) */
@@ -465,26 +720,46 @@ template <int N> struct Launch {
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
   }
-  template <int DIR> static int c2c_y(cube_ctx *c, FFTPlan *p, float2 *a) {
+  /* ncomp line sets (component stride NH*N*N) of strided line FFTs */
+  template <int DIR> static int strided(cube_ctx *c, FFTPlan *p, float2 *a, int64_t stride, int64_t ostride,
+                                        int ncomp) {
     constexpr int NH = N / 2 + 1;
-    dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
-    hipLaunchKernelGGL((k_c2c_s<N, DIR>), g, dim3(FFT_T), 0, c->stream, a, p->tw, NH, (int64_t)NH,
-                       (int64_t)NH * N);
-    CUBE_LAUNCH_CHECK();
+    const int64_t cstride = (int64_t)NH * N * N;
+    if constexpr (RC<N>::ok) {
+      dim3 g((NH + RC<N>::CB - 1) / RC<N>::CB, N, ncomp);
+      hipLaunchKernelGGL((k_c2c_r<N, DIR>), g, dim3(RC<N>::NT), 0, c->stream, a, p->tw, NH, stride, ostride,
+                         cstride);
+      CUBE_LAUNCH_CHECK();
+    } else {
+      dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
+      for (int d = 0; d < ncomp; ++d) {
+        hipLaunchKernelGGL((k_c2c_s<N, DIR>), g, dim3(FFT_T), 0, c->stream, a + d * cstride, p->tw, NH, stride,
+                           ostride);
+        CUBE_LAUNCH_CHECK();
+      }
+    }
     return CUBE_OK;
+  }
+  template <int DIR> static int c2c_y(cube_ctx *c, FFTPlan *p, float2 *a, int ncomp = 1) {
+    constexpr int NH = N / 2 + 1;
+    return strided<DIR>(c, p, a, (int64_t)NH, (int64_t)NH * N, ncomp);
   }
   template <int DIR> static int c2c_z(cube_ctx *c, FFTPlan *p, float2 *a) {
     constexpr int NH = N / 2 + 1;
-    dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
-    hipLaunchKernelGGL((k_c2c_s<N, DIR>), g, dim3(FFT_T), 0, c->stream, a, p->tw, NH,
-                       (int64_t)NH * N, (int64_t)NH);
-    CUBE_LAUNCH_CHECK();
-    return CUBE_OK;
+    return strided<DIR>(c, p, a, (int64_t)NH * N, (int64_t)NH, 1);
   }
   static int green(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int oct, float2 *work3) {
     constexpr int NH = N / 2 + 1;
-    dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
-    hipLaunchKernelGGL(k_green_z<N>, g, dim3(FFT_T), 0, c->stream, spec, work3, kern, oct, p->tw);
+    if constexpr (RC<N>::ok) {
+      dim3 g((NH + RC<N>::CB - 1) / RC<N>::CB, N);
+      if (oct)
+        hipLaunchKernelGGL((k_green_r<N, true>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw);
+      else
+        hipLaunchKernelGGL((k_green_r<N, false>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw);
+    } else {
+      dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
+      hipLaunchKernelGGL(k_green_z<N>, g, dim3(FFT_T), 0, c->stream, spec, work3, kern, oct, p->tw);
+    }
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
   }
@@ -562,8 +837,7 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
     if (t0 >= 0) tbeg(c);                                                         \
     if ((st = Launch<N>::green(c, p, spec, kern, kern_octant, work3))) return st; \
     if (t0 >= 0) { tend(c, t0); tbeg(c); }                                        \
-    for (int d = 0; d < 3; ++d)                                                   \
-      if ((st = Launch<N>::c2c_y<+1>(c, p, work3 + d * plane))) return st;        \
+    if ((st = Launch<N>::c2c_y<+1>(c, p, work3, 3))) return st;                   \
     if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                    \
     if ((st = Launch<N>::c2r3(c, p, work3, force3, div, (unsigned *)c->ured))) return st; \
     if (t0 >= 0) tend(c, t0 + 2);                                                 \

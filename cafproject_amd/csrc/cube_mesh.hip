@@ -172,9 +172,190 @@ __global__ void k_sum_parts1(const double *part, int nb, double *out) {
   }
 }
 
+/*
+ * Fine deposit, one thread per coarse cell C owning its 4x4x4 fine cells.
+ * Every particle that can reach them lives in the 2x2x2 source cells
+ * {C-1,C} per axis.  The thread walks those cells in the reference's visit
+ * order -- z by (ilayer,k), then y, then x, then l -- and adds each hit into
+ * a per-thread LDS accumulator of its 64 fine cells.  Restricted to one fine
+ * cell this is exactly that cell's reference order, so the sums are
+ * bit-identical; each particle is read by the 8 threads whose blocks its
+ * cloud can touch instead of by all 64+ fine cells around it.
+ */
+#define FD_T 128
+template <int IX>
+__global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<IX>::T *__restrict__ xp,
+                                                          const int32_t *__restrict__ rhoc,
+                                                          const int64_t *__restrict__ cstart, Geo g, float mass,
+                                                          float *__restrict__ rho, double *__restrict__ part) {
+  __shared__ float acc[64 * FD_T]; /* element q of thread t at q*FD_T + t: conflict free */
+  const int nc = g.nc, nf = 4 * nc;
+  const int64_t nc3 = (int64_t)nc * nc * nc;
+  const int t = threadIdx.x;
+  double tsum = 0;
+  for (int64_t cg = blockIdx.x * (int64_t)FD_T + t; cg - t < nc3; cg += (int64_t)gridDim.x * FD_T) {
+    const bool live = cg < nc3;
+    for (int q = 0; q < 64; ++q) acc[q * FD_T + t] = 0.f;
+    if (live) {
+      const int gg[3] = {(int)(cg % nc), (int)((cg / nc) % nc), (int)(cg / ((int64_t)nc * nc))};
+      int T[3], C[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        T[a] = gg[a] / g.nt;
+        C[a] = gg[a] - T[a] * g.nt + 1; /* 1-based local coarse cell */
+      }
+      Src sz;
+      sz.n = 2; sz.c[0] = C[2] - 1; sz.c[1] = C[2];
+      order_z(sz, 2 - g.ncb);
+      for (int qz = 0; qz < 2; ++qz)
+        for (int sy = C[1] - 1; sy <= C[1]; ++sy)
+          for (int sx = C[0] - 1; sx <= C[0]; ++sx) {
+            const int ci[3] = {sx, sy, sz.c[qz]};
+            const int64_t cc = cid_global(g, T[0] * g.nt + ci[0] - 1, T[1] * g.nt + ci[1] - 1,
+                                          T[2] * g.nt + ci[2] - 1);
+            const int np = rhoc[cc];
+            const int64_t p0 = cstart[cc];
+            for (int l = 0; l < np; ++l) {
+              int r[3];
+              float d1[3];
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                const int32_t code = (int32_t)xp[3 * (p0 + l) + a];
+                /* ncell*(i-1)+ncell*(x offset) rounded to real(4) (W/pm.f90:63):
+                   both terms are exact floats, so one float add rounds it */
+                const float off4 = (float)(4.0 * cube_xoff(IX, code));
+                const float tempx = (float)(4 * (ci[a] - 1)) + off4;
+                const int i1 = (int)floorf(tempx) + 1;
+                d1[a] = (float)i1 - tempx;
+                r[a] = i1 - 4 * (C[a] - 1) - 1; /* 0-based in C's block */
+              }
+#pragma unroll
+              for (int cz = 0; cz < 2; ++cz) {
+                const int z = r[2] + cz;
+                if (z < 0 || z > 3) continue;
+                const float wz = cz ? 1.0f - d1[2] : d1[2];
+#pragma unroll
+                for (int cy = 0; cy < 2; ++cy) {
+                  const int y = r[1] + cy;
+                  if (y < 0 || y > 3) continue;
+                  const float wy = cy ? 1.0f - d1[1] : d1[1];
+#pragma unroll
+                  for (int cx = 0; cx < 2; ++cx) {
+                    const int x = r[0] + cx;
+                    if (x < 0 || x > 3) continue;
+                    const float wx = cx ? 1.0f - d1[0] : d1[0];
+                    float &s = acc[((z * 4 + y) * 4 + x) * FD_T + t];
+                    s = s + wx * wy * wz * mass;
+                  }
+                }
+              }
+            }
+          }
+      const int64_t f0x = 4 * (int64_t)gg[0], f0y = 4 * (int64_t)gg[1], f0z = 4 * (int64_t)gg[2];
+      for (int z = 0; z < 4; ++z)
+        for (int y = 0; y < 4; ++y) {
+          float4 v;
+          v.x = acc[((z * 4 + y) * 4 + 0) * FD_T + t];
+          v.y = acc[((z * 4 + y) * 4 + 1) * FD_T + t];
+          v.z = acc[((z * 4 + y) * 4 + 2) * FD_T + t];
+          v.w = acc[((z * 4 + y) * 4 + 3) * FD_T + t];
+          *reinterpret_cast<float4 *>(rho + ((f0z + z) * nf + (f0y + y)) * nf + f0x) = v;
+          tsum += ((double)v.x + (double)v.y) + ((double)v.z + (double)v.w);
+        }
+    }
+  }
+  __syncthreads();
+  double *red = reinterpret_cast<double *>(acc);
+  red[t] = tsum;
+  __syncthreads();
+  for (int o = FD_T / 2; o > 0; o >>= 1) {
+    if (t < o) red[t] += red[t + o];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = red[0];
+}
+
+/*
+ * Coarse deposit, one thread per 2x2x2 block of mesh cells (m0..m0+1 per
+ * axis, within one tile since nt is even): the sources are the 4 cells
+ * m0-1..m0+2 per axis, walked in the reference order (ilayer=k mod 3, k,
+ * j, i, l), with the 8 sums in registers.  A particle that misses a cell
+ * adds +0.0, which leaves a sum of non-negative terms unchanged.
+ */
+template <int IX>
+__global__ __launch_bounds__(MT) void k_coarse_deposit_b2(const typename IntOf<IX>::T *__restrict__ xp,
+                                                          const int32_t *__restrict__ rhoc,
+                                                          const int64_t *__restrict__ cstart, Geo g, float mass,
+                                                          float *__restrict__ r3, double *__restrict__ part) {
+  const int nc = g.nc, nb = nc / 2;
+  const int64_t nb3 = (int64_t)nb * nb * nb;
+  double tsum = 0;
+  for (int64_t b = blockIdx.x * (int64_t)MT + threadIdx.x; b < nb3; b += (int64_t)gridDim.x * MT) {
+    const int gg[3] = {2 * (int)(b % nb), 2 * (int)((b / nb) % nb), 2 * (int)(b / ((int64_t)nb * nb))};
+    int T[3], m0[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      T[a] = gg[a] / g.nt;
+      m0[a] = gg[a] - T[a] * g.nt + 1;
+    }
+    int zs[4] = {m0[2] - 1, m0[2], m0[2] + 1, m0[2] + 2};
+    /* order by (k mod 3, k): insertion sort of 4 */
+    for (int a = 1; a < 4; ++a) {
+      const int x = zs[a];
+      int bb = a - 1;
+      while (bb >= 0 && ((zs[bb] % 3) > (x % 3) || ((zs[bb] % 3) == (x % 3) && zs[bb] > x))) {
+        zs[bb + 1] = zs[bb];
+        --bb;
+      }
+      zs[bb + 1] = x;
+    }
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int qz = 0; qz < 4; ++qz)
+      for (int sy = m0[1] - 1; sy <= m0[1] + 2; ++sy)
+        for (int sx = m0[0] - 1; sx <= m0[0] + 2; ++sx) {
+          const int ci[3] = {sx, sy, zs[qz]};
+          const int64_t cc = cid_global(g, T[0] * g.nt + ci[0] - 1, T[1] * g.nt + ci[1] - 1, T[2] * g.nt + ci[2] - 1);
+          const int np = rhoc[cc];
+          const int64_t p0 = cstart[cc];
+          for (int l = 0; l < np; ++l) {
+            float w0[3], w1[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const int32_t code = (int32_t)xp[3 * (p0 + l) + a];
+              const float tempx = (float)(((double)(ci[a] - 1) + cube_xoff(IX, code)) - 0.5);
+              const int i1 = (int)floorf(tempx) + 1;
+              const float d1 = (float)i1 - tempx, d2 = 1.0f - d1;
+              const int r = i1 - m0[a]; /* corner 1 relative to the block */
+              w0[a] = r == 0 ? d1 : (r == -1 ? d2 : 0.f);
+              w1[a] = r == 1 ? d1 : (r == 0 ? d2 : 0.f);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const float wx = (q & 1) ? w1[0] : w0[0], wy = (q & 2) ? w1[1] : w0[1], wz = (q & 4) ? w1[2] : w0[2];
+              s[q] = s[q] + wx * wy * wz * mass;
+            }
+          }
+        }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int x = gg[0] + (q & 1), y = gg[1] + ((q >> 1) & 1), z = gg[2] + ((q >> 2) & 1);
+      r3[((int64_t)z * nc + y) * nc + x] = s[q];
+      tsum += (double)s[q];
+    }
+  }
+  __shared__ double red[MT];
+  red[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int o = MT / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 #define DEP_BLOCKS 2048
 template <int IX> static int fine_dep(cube_ctx *c, float mass, double *t) {
-  hipLaunchKernelGGL(k_fine_deposit<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
+  hipLaunchKernelGGL(k_fine_deposit_cc<IX>, dim3(DEP_BLOCKS), dim3(FD_T), 0, c->stream,
                      (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_f, c->dred);
   CUBE_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, c->dred, DEP_BLOCKS, c->dred + 4000);
@@ -189,8 +370,12 @@ int cube_fine_deposit(cube_ctx *c, float mass_p, double *testrhof) {
   return c->p.izipx == 1 ? fine_dep<1>(c, mass_p, testrhof) : fine_dep<2>(c, mass_p, testrhof);
 }
 template <int IX> static int coarse_dep(cube_ctx *c, float mass, double *t) {
-  hipLaunchKernelGGL(k_coarse_deposit<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
-                     (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, c->dred);
+  if (c->nt % 2 == 0)
+    hipLaunchKernelGGL(k_coarse_deposit_b2<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
+                       (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, c->dred);
+  else
+    hipLaunchKernelGGL(k_coarse_deposit<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
+                       (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, c->dred);
   CUBE_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, c->dred, DEP_BLOCKS, c->dred + 4000);
   CUBE_LAUNCH_CHECK();

@@ -78,3 +78,31 @@ def test_host_timestep_matches_quadrature():
     t_q, _ = quad(lambda a: 1.0 / ((2.0 / 3.0) * a ** 1.5 * np.sqrt(1 + ol / om * a ** 3)), 0.02, 1.0)
     assert abs(float(c.a) - 1.0) < 2e-6
     assert abs(float(c.t) / t_q - 1) < 2e-4
+
+
+def test_fortran_interface_layouts_match_header():
+    """The bind(C) derived types of cafproject_amd/fortran/cube_c.f90 have
+    the sizes of the C structs (checked with amdflang's c_sizeof)."""
+    from cafproject_amd import cube
+    fc = "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("amdflang not installed")
+    prog = """
+program t
+  use iso_c_binding
+  use cube_c
+  type(cube_params) :: p
+  type(cube_vstats) :: v
+  type(cube_pm_out) :: o
+  print '(3i6)', c_sizeof(p), c_sizeof(v), c_sizeof(o)
+end program
+"""
+    src = os.path.join(ROOT, "cafproject_amd", "fortran", "cube_c.f90")
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "t.f90"), "w").write(prog)
+        exe = os.path.join(d, "t")
+        libdir = os.path.join(ROOT, "cafproject_amd")
+        subprocess.run([fc, "-module-dir", d, src, os.path.join(d, "t.f90"), "-o", exe, "-L" + libdir,
+                        "-lcube_hip", "-Wl,-rpath," + libdir], check=True, capture_output=True)
+        got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [ctypes.sizeof(cube.Params), ctypes.sizeof(cube.VStats), ctypes.sizeof(cube.PMOut)]
