@@ -102,12 +102,35 @@ def stage_bytes(np_, nf3, nc3, bx, bv):
 
 
 MAIN_KERNEL = {
-    "drift": "k_drift_dest/k_place/k_sort_small/k_drift_final",
-    "pp": "k_pp", "fine_deposit": "k_fine_deposit", "fine_kick": "k_kick<fine>",
-    "coarse_deposit": "k_coarse_deposit", "coarse_kick": "k_kick<coarse>",
-    "fine_fft": "k_r2c_x+k_c2c_s", "fine_inverse": "k_green_z+k_c2c_s+k_c2r_x3",
+    "drift": "k_drift_dest+k_place_ranked+k_sort_small+k_drift_final2",
+    "pp": "k_pp_blk", "fine_deposit": "k_fine_deposit_cc", "fine_kick": "k_kick<fine>",
+    "coarse_deposit": "k_coarse_deposit_b2", "coarse_kick": "k_kick<coarse>",
+    "fine_fft": "k_r2c_x+k_c2c_r", "fine_inverse": "k_green_r+k_c2c_r+k_c2r_x3",
     "coarse_force": "coarse fft kernels",
 }
+# kernel-name prefixes in the rocprofv3 PMC traffic table, per single-kernel stage
+PMC_NAMES = {"pp": ["k_pp_blk<"], "fine_deposit": ["k_fine_deposit_cc<"], "fine_kick": ["k_kick<"],
+             "coarse_deposit": ["k_coarse_deposit_b2<"], "coarse_kick": ["k_kick<"]}
+PMC_TABLE = os.path.join(ROOT, "profiles", "pmpp512_step_pmc_traffic.md")
+
+
+def pmc_traffic(stage, nc):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE table (measured on the 512^3 workload), or None."""
+    if nc != 256 or stage not in PMC_NAMES or not os.path.exists(PMC_TABLE):
+        return None
+    rows = {}
+    for line in open(PMC_TABLE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0].startswith("`"):
+            rows[cells[0].strip("`")] = float(cells[5]) * 1e9
+    fine = stage == "fine_kick"
+    for name, b in rows.items():
+        if any(name.startswith(p) for p in PMC_NAMES[stage]):
+            if name.startswith("k_kick<") and name.endswith("true>") != fine:
+                continue
+            return b
+    return None
 
 
 def cpu_baseline(seconds_cap=60.0):
@@ -150,12 +173,25 @@ def main():
     ap.add_argument("--box", type=float, default=400.0)
     ap.add_argument("--no-pp", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy kernel (torch), the reference "
+                         "point for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU (the whole box on one device); "
               f"--gpus {args.gpus} is not supported. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     from cafproject_amd.cube import Cube
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 2 x 1 GiB read + 1 GiB written")
+        del src, dst
+        torch.cuda.empty_cache()
     nc, nnt = args.nc, args.nnt
     izipx, izipv = 1, 2
     pp_on = not args.no_pp
@@ -211,7 +247,8 @@ def main():
                    "izipx": izipx, "izipv": izipv},
         "roofline": {"bound": "hbm", "kernel": MAIN_KERNEL[dom], "stage": dom,
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": pmc_traffic(dom, nc),
+                     "traffic_source": os.path.relpath(PMC_TABLE, ROOT) if pmc_traffic(dom, nc) else None,
                      "algorithmic_bytes_per_launch": d["bytes"], "avg_launch_ms": d["ms"]},
         "step_roofline": {"B_step": B_step, "achieved_GBps": B_step / (ms_step * 1e-3) / 1e9,
                           "frac": B_step / (ms_step * 1e-3) / HBM_PEAK},

@@ -52,8 +52,14 @@ def write(opath, z, header, xp, vp, rhoc, vfield, pid=None, image=1):
     h = np.array(header, HEADER_DTYPE)
     h["nplocal"] = xp.shape[0]
     h.tofile(fn["info"])
-    np.ascontiguousarray(xp).tofile(fn["xp"])
-    np.ascontiguousarray(vp).tofile(fn["vp"])
+    # the files hold izipx/izipv bytes per component, whatever the caller's dtype
+    xt = np.int8 if h["izipx"] == 1 else np.int16
+    vt = np.int8 if h["izipv"] == 1 else np.int16
+    for name, a, t in (("xp", xp, xt), ("vp", vp, vt)):
+        a = np.asarray(a)
+        if a.size and (a.min() < np.iinfo(t).min or a.max() > np.iinfo(t).max):
+            raise ValueError(f"{name}: values outside {np.dtype(t).name}")
+        np.ascontiguousarray(a, t).tofile(fn[name])
     np.ascontiguousarray(rhoc, np.int32).tofile(fn["np"])
     np.ascontiguousarray(vfield, np.float32).tofile(fn["vc"])
     if pid is not None:

@@ -27,10 +27,10 @@ extern "C" const char *cube_strerror(int code) {
 
 static void free_all(cube_ctx *c) {
   void *ptrs[] = {c->xp, c->vp, c->xp2, c->vp2, c->pid, c->pid2, c->rhoc, c->rhoc2, c->vfield, c->vfield2,
-                  c->cstart, c->cstart2, c->dest, c->key, c->sortbuf, c->fill, c->xdest, c->xkey,
+                  c->cstart, c->cstart2, c->dest, c->key, c->sortbuf, c->fill, c->xdest, c->xkey, c->rank, c->xrank, c->fbflag,
                   c->tanf_tab, c->vdec0, c->vdec1, c->rho_f, c->spec_f, c->specd_f, c->force_f,
                   c->kern_f, c->rho_c, c->spec_c, c->specd_c, c->force_c, c->kern_c, c->dred,
-                  c->ured, c->ired, c->scanbuf};
+                  c->ured, c->ired, c->scanbuf, c->vpart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   cube_fft_destroy(c->plan_f);
@@ -117,26 +117,30 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->fill, c->ncells);
   st |= dalloc(&c->xdest, c->xcap);
   st |= dalloc(&c->xkey, c->xcap);
+  st |= dalloc(&c->rank, c->cap);
+  st |= dalloc(&c->xrank, c->xcap);
+  st |= dalloc(&c->fbflag, c->ncells / 128 + 2);
   st |= dalloc(&c->tanf_tab, c->nvbin);
   st |= dalloc(&c->vdec0, c->nvbin);
   st |= dalloc(&c->vdec1, c->nvbin);
-  const int64_t nf3 = c->nf * c->nf * c->nf, nspec = c->nh * c->nf * c->nf;
+  const int64_t nf3 = c->nf * c->nf * c->nf, nspec = (int64_t)cube_spec_pitch((int)c->nf) * c->nf * c->nf;
   const int64_t ho = c->nf / 2 + 1;
   st |= dalloc(&c->rho_f, nf3);
   st |= dalloc(&c->spec_f, nspec);
   st |= dalloc(&c->specd_f, 3 * nspec);
   st |= dalloc(&c->force_f, 3 * nf3);
   st |= dalloc(&c->kern_f, 3 * ho * ho * ho);
-  const int64_t nc3 = c->ncells, nspec_c = (c->nc / 2 + 1) * c->nc * c->nc;
+  const int64_t nc3 = c->ncells, nspec_c = (int64_t)cube_spec_pitch((int)c->nc) * c->nc * c->nc;
   st |= dalloc(&c->rho_c, nc3);
   st |= dalloc(&c->spec_c, nspec_c);
   st |= dalloc(&c->specd_c, 3 * nspec_c);
   st |= dalloc(&c->force_c, 3 * nc3);
-  st |= dalloc(&c->kern_c, 3 * nspec_c);
+  st |= dalloc(&c->kern_c, 3 * (c->nc / 2 + 1) * c->nc * c->nc); /* unpadded [d][kz][ky][kx] */
   st |= dalloc(&c->dred, 4096);
   st |= dalloc(&c->ured, 64);
   st |= dalloc(&c->ired, 4096);
   st |= dalloc(&c->scanbuf, c->ncells / 1024 + 4);
+  st |= dalloc(&c->vpart, 3 * (c->ncells / 128 + 2));
   if (st) {
     free_all(c);
     delete c;

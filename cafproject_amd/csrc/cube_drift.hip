@@ -185,8 +185,9 @@ __global__ __launch_bounds__(DT) void k_drift_dest(const typename IntOf<IX>::T *
                                                    const int64_t *__restrict__ cstart,
                                                    const double *__restrict__ vdec, DriftP P,
                                                    int32_t *__restrict__ dest, uint64_t *__restrict__ key,
-                                                   int32_t *__restrict__ cnt, int32_t *__restrict__ xdest,
-                                                   uint64_t *__restrict__ xkey, int64_t xcap,
+                                                   int32_t *__restrict__ cnt, int32_t *__restrict__ rank,
+                                                   int32_t *__restrict__ xdest, uint64_t *__restrict__ xkey,
+                                                   int32_t *__restrict__ xrank, int64_t xcap,
                                                    unsigned long long *__restrict__ counters) {
   /* counters: [0] extra copies, [1] lost, [2] error flags */
   const Geo g = P.g;
@@ -257,15 +258,19 @@ __global__ __launch_bounds__(DT) void k_drift_dest(const typename IntOf<IX>::T *
                                             dt_[0] + g.nnt * (dt_[1] + g.nnt * dt_[2]));
           const uint64_t kk = ((uint64_t)il << 55) | ((uint64_t)(r[2] & 31) << 50) |
                               ((uint64_t)(r[1] & 31) << 45) | ((uint64_t)(r[0] & 31) << 40) | (uint64_t)p;
-          atomicAdd(cnt + dc, 1);
+          /* the count's old value is a free slot in the cell (arrival
+             order; the per-cell sort restores the reference order) */
+          const int32_t rk = atomicAdd(cnt + dc, 1);
           if (w == 0) {
             dest[p] = dc;
             key[p] = kk;
+            rank[p] = rk;
           } else {
             const unsigned long long x = atomicAdd(counters + 0, 1ull);
             if ((int64_t)x < xcap) {
               xdest[x] = dc;
               xkey[x] = kk;
+              xrank[x] = rk;
             } else {
               err |= DERR_XOVER;
             }
@@ -286,14 +291,41 @@ __global__ void k_place(const int32_t *__restrict__ dest, const uint64_t *__rest
   }
 }
 
+/* scatter to the slot taken at counting time: no second round of atomics */
+__global__ void k_place_ranked(const int32_t *__restrict__ dest, const int32_t *__restrict__ rank,
+                               const uint64_t *__restrict__ key, int64_t n, const int64_t *__restrict__ start,
+                               uint64_t *__restrict__ out) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t d = dest[p];
+    if (d < 0) continue;
+    out[start[d] + rank[p]] = key[p];
+  }
+}
+
 /* ---------------- per-cell sort ---------------- */
 #define SMALL_SORT 48
 #define BIG_SORT 16384
+#define FB_CELLS 128     /* cells per finalize workgroup */
+#define FB_CH 2048       /* slots staged in LDS per finalize pass */
+#define FB_LDS_SORT 64   /* largest cell sorted inside the finalize workgroup */
+/* flag[b] = 1 when finalize block b (cells b*FB_CELLS...) holds at most FB_CH
+   particles and no cell above FB_LDS_SORT: it sorts its cells in LDS */
+__global__ void k_fb_flags(const int32_t *__restrict__ cnt, const int64_t *__restrict__ start, int64_t ncells,
+                           uint8_t *__restrict__ flag) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c0 = b * FB_CELLS;
+  if (c0 >= ncells) return;
+  const int64_t c1 = min(c0 + FB_CELLS, ncells);
+  int mx = 0;
+  for (int64_t c = c0; c < c1; ++c) mx = max(mx, (int)cnt[c]);
+  flag[b] = (start[c1] - start[c0] <= FB_CH && mx <= FB_LDS_SORT) ? 1 : 0;
+}
 __global__ void k_sort_small(const int32_t *__restrict__ cnt, const int64_t *__restrict__ start, int64_t ncells,
                              uint64_t *__restrict__ a, int32_t *__restrict__ biglist,
-                             unsigned long long *__restrict__ counters) {
+                             unsigned long long *__restrict__ counters, const uint8_t *__restrict__ skip) {
   const int64_t cidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (cidx >= ncells) return;
+  if (skip && skip[cidx / FB_CELLS]) return; /* sorted by its finalize workgroup */
   const int n = cnt[cidx];
   if (n <= 1) return;
   if (n > SMALL_SORT) {
@@ -413,6 +445,186 @@ __global__ void k_drift_final(const typename IntOf<IX>::T *__restrict__ xp, cons
   }
 }
 
+/*
+ * Finalize by slot ranges: a workgroup owns FB_CELLS consecutive new cells
+ * and the contiguous slot range of their particles.  Lanes gather the
+ * decoded velocities of consecutive slots into LDS (one lane per slot), one
+ * thread per cell then accumulates vfield_new serially in slot order (the
+ * reference's order), and lanes write the re-quantised x, v and pid of
+ * consecutive slots, so the stores are coalesced.  The velocity statistics
+ * of the new state (W/update_particle.f90:172-224) are reduced on the way:
+ * per workgroup in a fixed tree, then over workgroups in order.
+ */
+template <int IX, int IV>
+__global__ __launch_bounds__(256) void k_drift_final2(
+    const typename IntOf<IX>::T *__restrict__ xp, const typename IntOf<IV>::T *__restrict__ vp,
+    const int32_t *__restrict__ pid, const float *__restrict__ vfield_old, const int64_t *__restrict__ start,
+    const uint64_t *__restrict__ keys, const uint8_t *__restrict__ flag, const double *__restrict__ vdec, DriftP P,
+    double venc, typename IntOf<IX>::T *__restrict__ xp2, typename IntOf<IV>::T *__restrict__ vp2,
+    int32_t *__restrict__ pid2, float *__restrict__ vfield_new, int64_t ncells, double *__restrict__ part) {
+  __shared__ int64_t cst[FB_CELLS + 1];
+  __shared__ double vr[FB_CH][3];
+  __shared__ uint64_t pix[FB_CH]; /* sorted keys, then source particle indices */
+  __shared__ float vn[FB_CELLS][3];
+  __shared__ double red[3][256];
+  const Geo g = P.g;
+  const int nt = g.nt, ncb = g.ncb, tid = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * FB_CELLS;
+  const int ncl = (int)min((int64_t)FB_CELLS, ncells - c0);
+  for (int i = tid; i <= ncl; i += 256) cst[i] = start[c0 + i];
+  __syncthreads();
+  const int64_t S0 = cst[0], S1 = cst[ncl];
+  /* every slot fits (and every cell is small): sort the keys of each cell
+     here and keep the gathers in LDS for the write pass */
+  const bool one = flag[blockIdx.x] != 0;
+  if (one) {
+    for (int64_t s = S0 + tid; s < S1; s += 256) pix[s - S0] = keys[s];
+    __syncthreads();
+    if (tid < ncl) {
+      /* insertion sort of the cell's keys (unique: they carry the particle index) */
+      const int lo = (int)(cst[tid] - S0), hi = (int)(cst[tid + 1] - S0);
+      for (int q = lo + 1; q < hi; ++q) {
+        const uint64_t x = pix[q];
+        int r = q - 1;
+        while (r >= lo && pix[r] > x) {
+          pix[r + 1] = pix[r];
+          --r;
+        }
+        pix[r + 1] = x;
+      }
+    }
+    __syncthreads();
+  }
+
+  /* slot -> (local cell, source particle, decoded velocity) */
+  auto gather = [&](int64_t s, int &lc, int64_t &p, double v[3]) {
+    int lo = 0, hi = ncl;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (cst[mid] <= s) lo = mid; else hi = mid;
+    }
+    lc = lo;
+    const int64_t cidx = c0 + lo;
+    const int i = (int)(cidx % nt), j = (int)((cidx / nt) % nt), k = (int)((cidx / ((int64_t)nt * nt)) % nt);
+    const int t = (int)(cidx / ((int64_t)nt * nt * nt));
+    const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
+    const uint64_t kk = one ? pix[s - S0] : keys[s];
+    p = (int64_t)(kk & KEY_PMASK);
+    const int rx = (int)((kk >> 40) & 31), ry = (int)((kk >> 45) & 31), rz = (int)((kk >> 50) & 31);
+    /* source cell = destination + (r - 2ncb), in the destination tile frame */
+    const int64_t sc = cid_global(g, tx * nt + i + rx - 2 * ncb, ty * nt + j + ry - 2 * ncb,
+                                  tz * nt + k + rz - 2 * ncb);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vfield_old[3 * sc + a];
+  };
+
+  float acc[3] = {0.f, 0.f, 0.f};
+  if (tid < ncl) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = (float)((double)vfield_old[3 * (c0 + tid) + a] * WEIGHT_V);
+  }
+  for (int64_t ch = S0; ch < S1; ch += FB_CH) {
+    const int64_t ce = min(ch + FB_CH, S1);
+    for (int64_t s = ch + tid; s < ce; s += 256) {
+      int lc;
+      int64_t p;
+      double v[3];
+      gather(s, lc, p, v);
+      vr[s - ch][0] = v[0];
+      vr[s - ch][1] = v[1];
+      vr[s - ch][2] = v[2];
+      pix[s - ch] = (uint64_t)p;
+    }
+    __syncthreads();
+    if (tid < ncl) {
+      const int64_t lo = max(cst[tid], ch), hi = min(cst[tid + 1], ce);
+      for (int64_t q = lo; q < hi; ++q) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) acc[a] = (float)((double)acc[a] + vr[q - ch][a]);
+      }
+    }
+    __syncthreads();
+  }
+  double sc = 0, sr = 0, ss = 0;
+  if (tid < ncl) {
+    const double n = (double)(cst[tid + 1] - cst[tid]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      vn[tid][a] = (float)((double)acc[a] / (n + WEIGHT_V));
+      vfield_new[3 * (c0 + tid) + a] = vn[tid][a];
+    }
+    const float v0 = vn[tid][0], v1 = vn[tid][1], v2 = vn[tid][2];
+    sc = (double)((v0 * v0 + v1 * v1) + v2 * v2);
+  }
+  __syncthreads();
+  for (int64_t s = S0 + tid; s < S1; s += 256) {
+    int lc;
+    int64_t p;
+    double v[3];
+    if (one) {
+      int lo = 0, hi = ncl;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cst[mid] <= s) lo = mid; else hi = mid;
+      }
+      lc = lo;
+      p = (int64_t)pix[s - S0];
+      v[0] = vr[s - S0][0];
+      v[1] = vr[s - S0][1];
+      v[2] = vr[s - S0][2];
+    } else {
+      gather(s, lc, p, v);
+    }
+    double r[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      /* xp_new=xp+nint(dt_mid*vreal/(x_resolution*ncell)) (:115) */
+      const double qx = ((double)P.dt_mid * v[a]) / (P.xres * 4.0);
+      xp2[3 * s + a] = (typename IntOf<IX>::T)cube_wrap(IX, (int64_t)xp[3 * p + a] + (int64_t)cube_nint(qx));
+      /* vp_new=nint(...atan(...*(vreal-vfield_new))...) (:116-117) */
+      const int32_t code = cube_encode_v(IV, venc, v[a] - (double)vn[lc][a]);
+      vp2[3 * s + a] = (typename IntOf<IV>::T)code;
+      r[a] = vdec[(int32_t)(typename IntOf<IV>::T)code + P.nvh];
+    }
+    if (pid) pid2[s] = pid[p];
+    sr += (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+    r[0] += (double)vn[lc][0];
+    r[1] += (double)vn[lc][1];
+    r[2] += (double)vn[lc][2];
+    ss += (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+  }
+  red[0][tid] = sc;
+  red[1][tid] = sr;
+  red[2][tid] = ss;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o)
+      for (int a = 0; a < 3; ++a) red[a][tid] += red[a][tid + o];
+    __syncthreads();
+  }
+  if (tid < 3) part[3 * blockIdx.x + tid] = red[tid][0];
+}
+
+/* fixed-order sum of nb triples: thread t sums a contiguous run of blocks,
+   then a tree over the threads */
+__global__ __launch_bounds__(1024) void k_sum_parts_tree(const double *__restrict__ part, int64_t nb,
+                                                         double *__restrict__ out) {
+  __shared__ double red[3][1024];
+  const int t = threadIdx.x;
+  const int64_t per = (nb + 1023) / 1024, b0 = t * per, b1 = min(b0 + per, nb);
+  double s[3] = {0, 0, 0};
+  for (int64_t b = b0; b < b1; ++b)
+    for (int a = 0; a < 3; ++a) s[a] += part[3 * b + a];
+  for (int a = 0; a < 3; ++a) red[a][t] = s[a];
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o)
+      for (int a = 0; a < 3; ++a) red[a][t] += red[a][t + o];
+    __syncthreads();
+  }
+  if (t < 3) out[t] = red[t][0];
+}
+
 /* ---------------- velocity statistics (:172-224) ---------------- */
 #define VS_BLOCKS 1024
 template <int IV>
@@ -498,7 +710,8 @@ extern "C" int cube_sort_small_segments(cube_ctx *c, const int32_t *cnt, const i
   if (st) return st;
   const unsigned nbc = (unsigned)((c->ncells + 255) / 256);
   int32_t *biglist = (int32_t *)c->dest;
-  hipLaunchKernelGGL(k_sort_small, dim3(nbc), dim3(256), 0, c->stream, cnt, start, c->ncells, keys, biglist, ctr);
+  hipLaunchKernelGGL(k_sort_small, dim3(nbc), dim3(256), 0, c->stream, cnt, start, c->ncells, keys, biglist, ctr,
+                     (const uint8_t *)nullptr);
   CUBE_LAUNCH_CHECK();
   unsigned long long hc[8];
   CUBE_CHECK(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
@@ -520,12 +733,11 @@ template <int IX, int IV> static int drift_impl(cube_ctx *c, DriftP P, double ve
   unsigned long long *ctr = (unsigned long long *)c->ured;
   int st;
   if ((st = fill_i32(c, c->rhoc2, c->ncells, 0))) return st;
-  if ((st = fill_i32(c, c->fill, c->ncells, 0))) return st;
   if ((st = fill_i64(c, (int64_t *)ctr, 8, 0))) return st;
   const int64_t nrows = c->ncells / c->nt;
   hipLaunchKernelGGL((k_drift_dest<IX, IV>), dim3((unsigned)nrows), dim3(DT), 0, c->stream, (const TX *)c->xp,
-                     (const TV *)c->vp, c->vfield, c->cstart, c->vdec0, P, c->dest, c->key, c->rhoc2, c->xdest,
-                     c->xkey, c->xcap, ctr);
+                     (const TV *)c->vp, c->vfield, c->cstart, c->vdec0, P, c->dest, c->key, c->rhoc2, c->rank,
+                     c->xdest, c->xkey, c->xrank, c->xcap, ctr);
   CUBE_LAUNCH_CHECK();
   int64_t total = 0;
   if ((st = cube_scan_counts(c, c->rhoc2, c->cstart2, c->ncells, &total))) return st;
@@ -534,18 +746,22 @@ template <int IX, int IV> static int drift_impl(cube_ctx *c, DriftP P, double ve
   if (hc[2] & DERR_XOVER) return CUBE_ERR_IMAGE_OVERFLOW;
   if (total > c->cap) return CUBE_ERR_IMAGE_OVERFLOW;
   *nxtra = (int64_t)hc[0];
-  hipLaunchKernelGGL(k_place, dim3(4096), dim3(256), 0, c->stream, c->dest, c->key, c->nplocal, c->cstart2, c->fill,
-                     c->sortbuf);
+  hipLaunchKernelGGL(k_place_ranked, dim3(4096), dim3(256), 0, c->stream, c->dest, c->rank, c->key, c->nplocal,
+                     c->cstart2, c->sortbuf);
   CUBE_LAUNCH_CHECK();
   if (hc[0]) {
-    hipLaunchKernelGGL(k_place, dim3(64), dim3(256), 0, c->stream, c->xdest, c->xkey, (int64_t)hc[0], c->cstart2,
-                       c->fill, c->sortbuf);
+    hipLaunchKernelGGL(k_place_ranked, dim3(64), dim3(256), 0, c->stream, c->xdest, c->xrank, c->xkey,
+                       (int64_t)hc[0], c->cstart2, c->sortbuf);
     CUBE_LAUNCH_CHECK();
   }
   const unsigned nbc = (unsigned)((c->ncells + 255) / 256);
+  const int64_t nfb = (c->ncells + FB_CELLS - 1) / FB_CELLS;
+  hipLaunchKernelGGL(k_fb_flags, dim3((unsigned)((nfb + 255) / 256)), dim3(256), 0, c->stream, c->rhoc2, c->cstart2,
+                     c->ncells, c->fbflag);
+  CUBE_LAUNCH_CHECK();
   int32_t *biglist = (int32_t *)c->dest; /* dest is dead after placement */
   hipLaunchKernelGGL(k_sort_small, dim3(nbc), dim3(256), 0, c->stream, c->rhoc2, c->cstart2, c->ncells, c->sortbuf,
-                     biglist, ctr);
+                     biglist, ctr, c->fbflag);
   CUBE_LAUNCH_CHECK();
   CUBE_CHECK(hipMemcpy(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   if (hc[3]) {
@@ -555,9 +771,12 @@ template <int IX, int IV> static int drift_impl(cube_ctx *c, DriftP P, double ve
     CUBE_CHECK(hipMemcpy(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     if (hc[2] & DERR_BIG) return CUBE_ERR_INVALID;
   }
-  hipLaunchKernelGGL((k_drift_final<IX, IV>), dim3(nbc), dim3(256), 0, c->stream, (const TX *)c->xp, (const TV *)c->vp,
-                     c->p.has_pid ? c->pid : nullptr, c->vfield, c->rhoc2, c->cstart2, c->sortbuf, c->vdec0, P, venc,
-                     (TX *)c->xp2, (TV *)c->vp2, c->pid2, c->vfield2, c->ncells);
+  hipLaunchKernelGGL((k_drift_final2<IX, IV>), dim3((unsigned)nfb), dim3(256), 0, c->stream, (const TX *)c->xp,
+                     (const TV *)c->vp, c->p.has_pid ? c->pid : nullptr, c->vfield, c->cstart2, c->sortbuf,
+                     c->fbflag, c->vdec0, P, venc, (TX *)c->xp2, (TV *)c->vp2, c->pid2, c->vfield2, c->ncells,
+                     c->vpart);
+  CUBE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_parts_tree, dim3(1), dim3(1024), 0, c->stream, c->vpart, nfb, c->dred);
   CUBE_LAUNCH_CHECK();
   CUBE_CHECK(hipStreamSynchronize(c->stream));
   c->nplocal = total;
@@ -610,11 +829,10 @@ int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi
   int32_t *tr = c->rhoc; c->rhoc = c->rhoc2; c->rhoc2 = tr;
   float *tf = c->vfield; c->vfield = c->vfield2; c->vfield2 = tf;
   int64_t *ts = c->cstart; c->cstart = c->cstart2; c->cstart2 = ts;
-  /* std of the velocity field, FP64 sums (:171-224) */
+  /* std of the velocity field, FP64 sums (:171-224), reduced by the
+     finalize kernel of the drift */
   double s3[3];
-  tbeg(c);
-  if ((st = cube_vstats_run(c, sigma_vi, s3))) return st;
-  tend(c, CUBE_T_VSTATS);
+  CUBE_CHECK(hipMemcpy(s3, c->dred, 3 * sizeof(double), hipMemcpyDeviceToHost));
   const double npg = (double)c->npglobal;
   cube_vstats v;
   memset(&v, 0, sizeof(v));

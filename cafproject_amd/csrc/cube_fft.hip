@@ -420,23 +420,30 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
                                                      const float *__restrict__ kern,
                                                      const float2 *__restrict__ tw) {
   using G = RC<N>;
-  constexpr int NH = N / 2 + 1, H = N / 2, R1 = G::R1, R2 = G::R2, H1 = H + 1;
-  constexpr int64_t stride = (int64_t)N * NH, plane = (int64_t)NH * N * N;
+  constexpr int NH = N / 2 + 1, H = N / 2, R1 = G::R1, R2 = G::R2, H1 = H + 1, SP = cube_spec_pitch(N);
+  constexpr int64_t stride = (int64_t)N * SP, plane = (int64_t)SP * N * N;
+  /* the forward z spectrum is needed by all three inverse transforms; at
+     N=1024 (512 threads, 256 registers each) its upper half waits in LDS
+     instead of registers, which would otherwise spill */
+  constexpr int KL = N >= 1024 ? R2 / 2 : 0;
   __shared__ float S[G::CB * G::SC];
   __shared__ float2 twl[N];
+  __shared__ float2 keep[KL > 0 ? KL * G::NT : 1];
   const int tid = threadIdx.x, c = tid % G::CB, t = tid / G::CB;
   for (int e = tid; e < N; e += G::NT) twl[e] = tw[e];
   const int kx0 = blockIdx.x * G::CB + c, y = blockIdx.y;
   const bool ok = kx0 < NH;
   const int kx = ok ? kx0 : NH - 1;
   const uint32_t lo = (uint32_t)(((int64_t)t * stride + kx) * 8);
-  const float2 *src = spec + (int64_t)y * NH;
+  const float2 *src = spec + (int64_t)y * SP;
   float2 v[R2];
   const int s0 = opaque_zero();
 #pragma unroll
   for (int m = 0; m < R2; ++m) v[m] = *boff(uni(src + (int64_t)(R1 * m) * stride, s0), lo);
   __syncthreads();
   blk_fft<N, -1>(v, c, t, S, twl);
+#pragma unroll
+  for (int m = R2 - KL; m < R2; ++m) keep[(m - (R2 - KL)) * G::NT + tid] = v[m];
   const int ky = y <= H ? y : N - y;
   const float sy = y <= H ? 1.f : -1.f;
 #pragma unroll 1
@@ -446,6 +453,7 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
 #pragma unroll
     for (int m = 0; m < R2; ++m) {
       const int n = t + R1 * m;
+      const float2 vm = m < R2 - KL ? v[m] : keep[(m - (R2 - KL)) * G::NT + tid];
       float K;
       if constexpr (OCT) {
         const int kz = n <= H ? n : N - n;
@@ -458,11 +466,11 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
         K = *boff(kb, (uint32_t)((n * N * NH + kx) * 4));
       }
       /* rho_f(::2)=-crho(2::2)*K, rho_f(2::2)=crho(::2)*K  (W/pm.f90:119-120) */
-      w[m] = make_float2((-v[m].y) * K, v[m].x * K);
+      w[m] = make_float2((-vm.y) * K, vm.x * K);
     }
     blk_fft<N, +1>(w, c, t, S, twl);
     if (ok) {
-      float2 *dst = outd + d * plane + (int64_t)y * NH;
+      float2 *dst = outd + d * plane + (int64_t)y * SP;
       const int s1 = opaque_zero();
 #pragma unroll
       for (int m = 0; m < R2; ++m) *boff(uni(dst + (int64_t)(R1 * m) * stride, s1), lo) = w[m];
@@ -495,7 +503,7 @@ __global__ __launch_bounds__(FFT_T) void k_r2c_x(const float *__restrict__ in, f
     const float2 o = make_float2(0.5f * (zk.y + zc.y), -0.5f * (zk.x - zc.x));
     const float2 w = tw[k];
     const float2 wo = cmul(w, o);
-    if (row0 + l < nrows) out[(row0 + l) * (L + 1) + k] = cadd(e, wo);
+    if (row0 + l < nrows) out[(row0 + l) * cube_spec_pitch(N) + k] = cadd(e, wo);
   }
 }
 
@@ -530,14 +538,14 @@ template <int N>
 __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ spec, float2 *__restrict__ outd,
                                                  const float *__restrict__ kern, int octant,
                                                  const float2 *__restrict__ tw) {
-  constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = N + 1, NH = N / 2 + 1, H = N / 2;
+  constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = N + 1, NH = N / 2 + 1, H = N / 2, SP = cube_spec_pitch(N);
   constexpr int EPT = (B * N + FFT_T - 1) / FFT_T;
   __shared__ float2 sm[B * NP];
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int col0 = blockIdx.x * B, y = blockIdx.y;
-  const int64_t stride = (int64_t)N * NH;   /* between z planes */
-  const int64_t plane = (int64_t)NH * N * N;
-  const float2 *src = spec + (int64_t)y * NH + col0;
+  const int64_t stride = (int64_t)N * SP;   /* between z planes */
+  const int64_t plane = (int64_t)SP * N * N;
+  const float2 *src = spec + (int64_t)y * SP + col0;
   for (int idx = tid; idx < B * N; idx += FFT_T) {
     const int l = idx % B, n = idx / B;
     sm[l * NP + n] = (col0 + l < NH) ? src[n * stride + l] : make_float2(0.f, 0.f);
@@ -578,7 +586,7 @@ __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ sp
     }
     __syncthreads();
     line_fft<N, +1, TL>(sm + line * NP, tl, tw, N);
-    float2 *dst = outd + d * plane + (int64_t)y * NH + col0;
+    float2 *dst = outd + d * plane + (int64_t)y * SP + col0;
     for (int idx = tid; idx < B * N; idx += FFT_T) {
       const int l = idx % B, n = idx / B;
       if (col0 + l < NH) dst[n * stride + l] = sm[l * NP + n];
@@ -593,12 +601,12 @@ template <int N>
 __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ specd, float *__restrict__ force,
                                                 const float2 *__restrict__ tw, int64_t nrows, float div,
                                                 unsigned int *__restrict__ f2max) {
-  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = L + 1;
+  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC, TL = FFT_T / B, LP = L + 1;
   __shared__ float2 sm[3 * B * LP];
   __shared__ float red[FFT_T];
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int64_t row0 = (int64_t)blockIdx.x * B;
-  const int64_t plane = nrows * (L + 1);
+  const int64_t plane = nrows * cube_spec_pitch(N);
   for (int d = 0; d < 3; ++d) {
     const float2 *in = specd + d * plane;
     /* Z[k] = E + i O, E = X[k]+conj(X[L-k]), O = (X[k]-conj(X[L-k])) W^-k */
@@ -606,8 +614,8 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
       const int l = idx / L, k = idx % L;
       float2 z = make_float2(0.f, 0.f);
       if (row0 + l < nrows) {
-        const float2 xk = in[(row0 + l) * (L + 1) + k];
-        const float2 xc = in[(row0 + l) * (L + 1) + (L - k)];
+        const float2 xk = in[(row0 + l) * cube_spec_pitch(N) + k];
+        const float2 xc = in[(row0 + l) * cube_spec_pitch(N) + (L - k)];
         const float2 e = make_float2(xk.x + xc.x, xk.y - xc.y);
         float2 w = tw[k];
         w.y = -w.y;
@@ -662,8 +670,8 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x1(const float2 *__restrict__ spe
     const int l = idx / L, k = idx % L;
     float2 z = make_float2(0.f, 0.f);
     if (row0 + l < nrows) {
-      const float2 xk = spec[(row0 + l) * (L + 1) + k];
-      const float2 xc = spec[(row0 + l) * (L + 1) + (L - k)];
+      const float2 xk = spec[(row0 + l) * cube_spec_pitch(N) + k];
+      const float2 xc = spec[(row0 + l) * cube_spec_pitch(N) + (L - k)];
       const float2 e = make_float2(xk.x + xc.x, xk.y - xc.y);
       float2 w = tw[k];
       w.y = -w.y;
@@ -724,7 +732,7 @@ template <int N> struct Launch {
   template <int DIR> static int strided(cube_ctx *c, FFTPlan *p, float2 *a, int64_t stride, int64_t ostride,
                                         int ncomp) {
     constexpr int NH = N / 2 + 1;
-    const int64_t cstride = (int64_t)NH * N * N;
+    const int64_t cstride = (int64_t)cube_spec_pitch(N) * N * N;
     if constexpr (RC<N>::ok) {
       dim3 g((NH + RC<N>::CB - 1) / RC<N>::CB, N, ncomp);
       hipLaunchKernelGGL((k_c2c_r<N, DIR>), g, dim3(RC<N>::NT), 0, c->stream, a, p->tw, NH, stride, ostride,
@@ -742,11 +750,11 @@ template <int N> struct Launch {
   }
   template <int DIR> static int c2c_y(cube_ctx *c, FFTPlan *p, float2 *a, int ncomp = 1) {
     constexpr int NH = N / 2 + 1;
-    return strided<DIR>(c, p, a, (int64_t)NH, (int64_t)NH * N, ncomp);
+    return strided<DIR>(c, p, a, (int64_t)cube_spec_pitch(N), (int64_t)cube_spec_pitch(N) * N, ncomp);
   }
   template <int DIR> static int c2c_z(cube_ctx *c, FFTPlan *p, float2 *a) {
     constexpr int NH = N / 2 + 1;
-    return strided<DIR>(c, p, a, (int64_t)NH * N, (int64_t)NH, 1);
+    return strided<DIR>(c, p, a, (int64_t)cube_spec_pitch(N) * N, (int64_t)cube_spec_pitch(N), 1);
   }
   static int green(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int oct, float2 *work3) {
     constexpr int NH = N / 2 + 1;
@@ -764,7 +772,7 @@ template <int N> struct Launch {
     return CUBE_OK;
   }
   static int c2r3(cube_ctx *c, FFTPlan *p, const float2 *work3, float *force, float div, unsigned *f2) {
-    constexpr int BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC;
+    constexpr int B = Cfg<N>::BC;
     const int64_t rows = (int64_t)N * N;
     hipLaunchKernelGGL(k_c2r_x3<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0, c->stream, work3,
                        force, p->tw, rows, div, f2);
@@ -829,7 +837,7 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
                           float2 *work3, float *force3, float div, float *f2max, int t0) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, (unsigned *)c->ured, 1);
   CUBE_LAUNCH_CHECK();
-  const int64_t nh = p->n / 2 + 1, plane = nh * (int64_t)p->n * p->n;
+  const int64_t plane = (int64_t)cube_spec_pitch(p->n) * p->n * p->n;
   switch (p->n) {
 #define CASE(N)                                                                   \
   case N: {                                                                       \

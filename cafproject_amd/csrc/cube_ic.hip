@@ -59,7 +59,8 @@ __global__ void k_ic_filter(float2 *spec, int ng, double cell, const double *tab
       const double s = 2.0 * sin(0.5 * kr);
       kt2 += s * s;
     }
-    float2 v = spec[q];
+    const int64_t sq = ((int64_t)iz * ng + iy) * cube_spec_pitch(ng) + ix;
+    float2 v = spec[sq];
     if (q == 0 || kt2 == 0.0) {
       v = make_float2(0.f, 0.f);
     } else {
@@ -73,7 +74,7 @@ __global__ void k_ic_filter(float2 *spec, int ng, double cell, const double *tab
       const double amp = exp(0.5 * lp) / (cell * sqrt(cell)) * (-4.0 * 3.141592653589793 / kt2) * norm;
       v = make_float2((float)(v.x * amp), (float)(v.y * amp));
     }
-    spec[q] = v;
+    spec[sq] = v;
   }
 }
 
@@ -247,7 +248,7 @@ extern "C" int cube_set_state_xv(cube_ctx *c, int64_t n, const double *x, const 
   CUBE_CHECK(hipSetDevice(c->device));
   /* stage through the (idle) fine-mesh buffers */
   const size_t xb = (size_t)n * 3 * sizeof(double), vb = (size_t)n * 3 * sizeof(float);
-  const size_t nspec = (size_t)c->nh * c->nf * c->nf;
+  const size_t nspec = (size_t)cube_spec_pitch((int)c->nf) * c->nf * c->nf;
   if (xb > 3 * nspec * sizeof(float2) || vb > 3 * (size_t)c->nf * c->nf * c->nf * sizeof(float))
     return CUBE_ERR_INVALID;
   double *dx = (double *)c->specd_f;

@@ -10,7 +10,8 @@
  *    tile is read straight from the periodic neighbour's physical cell, so
  *    buffer_grid/buffer_x/buffer_v reduce to index arithmetic.
  *  - fine mesh: ONE periodic nf^3 mesh for the whole box (nf = 4 nc),
- *    x fastest; its half spectrum (nf/2+1) x nf x nf complex.
+ *    x fastest; its half spectrum nf x nf rows of nf/2+1 complex, padded
+ *    to cube_spec_pitch(nf).
  *  - coarse mesh nc^3 and its half spectrum.
  */
 #ifndef CUBE_INTERNAL_H
@@ -60,14 +61,16 @@ struct cube_ctx {
   int32_t *fill;            /* ncells */
   int32_t *xdest;           /* extra copies (duplicates in the reference)      */
   uint64_t *xkey;
+  int32_t *rank, *xrank;    /* slot of each (extra) copy inside its new cell, arrival order */
+  uint8_t *fbflag;          /* per finalize block: 1 = sorted and finished in LDS */
   int64_t xcap;
   /* codec tables */
   float *tanf_tab;          /* nvbin: tanf(pi*code/(nvbin-1)), index code+nvbin/2 */
   double *vdec0, *vdec1;    /* nvbin: decoded residual velocity for two sigmas  */
   /* fine mesh */
   float *rho_f;             /* nf^3 */
-  float2 *spec_f;           /* nh*nf*nf */
-  float2 *specd_f;          /* 3 * nh*nf*nf */
+  float2 *spec_f;           /* pitch*nf*nf */
+  float2 *specd_f;          /* 3 * pitch*nf*nf */
   float *force_f;           /* 3*nf^3, (d fastest) */
   float *kern_f;            /* 3*(nf/2+1)^3 octant */
   /* coarse mesh */
@@ -81,6 +84,7 @@ struct cube_ctx {
   uint64_t *ured;           /* 64 words */
   int64_t *ired;            /* 4096 */
   int64_t *scanbuf;         /* ncells/1024 + 2 block sums of the scan */
+  double *vpart;            /* 3 per finalize block of the drift (velocity sums) */
   /* host-side persistent state */
   float pp_ftemp;           /* running max of W/ext_pp_force.f90:123 */
   FFTPlan *plan_f, *plan_c;
@@ -132,6 +136,11 @@ __device__ static inline int64_t cid_global(const Geo &g, int gx, int gy, int gz
 template <int B> struct IntOf;
 template <> struct IntOf<1> { typedef int8_t T; };
 template <> struct IntOf<2> { typedef int16_t T; };
+
+/* Half spectra of an n^3 real mesh are stored as rows of n/2+1 complex
+   values padded to a multiple of 16 (128 bytes), so every row and every
+   16-column batch of the strided FFT passes starts on a cache line. */
+__host__ __device__ constexpr int cube_spec_pitch(int n) { return ((n / 2 + 1) + 15) / 16 * 16; }
 
 /* launchers (one file each) */
 int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi, cube_vstats *out);

@@ -559,11 +559,11 @@ __global__ void k_fine_kernel_rs(float *rs, int n, const float *tab, int d, int 
 }
 /* keep Im of the positive-frequency octant: kern[d][kz][ky][kx], 0..n/2 */
 __global__ void k_extract_octant(const float2 *spec, int n, float *kern, int d) {
-  const int h = n / 2 + 1, nh = n / 2 + 1;
+  const int h = n / 2 + 1;
   const int64_t tot = (int64_t)h * h * h;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
     const int kx = (int)(q % h), ky = (int)((q / h) % h), kz = (int)(q / ((int64_t)h * h));
-    kern[(int64_t)d * tot + q] = spec[((int64_t)kz * n + ky) * nh + kx].y;
+    kern[(int64_t)d * tot + q] = spec[((int64_t)kz * n + ky) * cube_spec_pitch(n) + kx].y;
   }
 }
 /* uncorrected coarse kernel -r/r^3 in fine units (W/kernel_c.f90:16-28) and
@@ -605,8 +605,9 @@ __global__ void k_lrckcorr(const float2 *spec_corr, const float2 *spec_plain, in
       ks[a] = 2.0f * sinf(CUBE_PI_F * kx[a] / (float)nc);
     }
     const float kr = sqrtf((kx[0] * kx[0] + kx[1] * kx[1]) + kx[2] * kx[2]);
-    const float kc = spec_corr[q].y;
-    const float cu = spec_plain[q].y;
+    const int64_t sq = ((int64_t)iz * nc + iy) * cube_spec_pitch(nc) + ix;
+    const float kc = spec_corr[sq].y;
+    const float cu = spec_plain[sq].y;
     float out = kc;
     /* for nc < 24 the |k|<=8 sphere reaches the Nyquist plane of axis d,
        where Im FFT of the odd kernel is exactly zero and the reference
@@ -639,7 +640,7 @@ int cube_green_init(cube_ctx *c, const float *wfxyzf, const float *wfxyzc) {
   }
   for (int d = 0; d < 3 && !st; ++d) {
     float *plain = c->rho_c, *corr = (float *)c->specd_c; /* specd_c is large enough for nc^3 floats */
-    float2 *sp_plain = c->spec_c, *sp_corr = c->specd_c + (nc / 2 + 1) * (int64_t)nc * nc;
+    float2 *sp_plain = c->spec_c, *sp_corr = c->specd_c + (int64_t)cube_spec_pitch(nc) * nc * nc;
     hipLaunchKernelGGL(k_coarse_kernel_rs, dim3(1024), dim3(256), 0, c->stream, plain, corr, nc, dctab, d);
     if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
     if (!st) st = cube_fft_forward_full(c, c->plan_c, plain, sp_plain);

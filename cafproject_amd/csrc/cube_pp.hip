@@ -16,7 +16,9 @@
  */
 #include "cube_internal.h"
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 
 #define PT 256
 
@@ -126,12 +128,262 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
   }
 }
 
+/*
+ * Blocked PP: one workgroup per 4x4x4 coarse cells of a tile.  The
+ * particles of the 6^3 coarse cells around it that fall in its fine-cell
+ * neighbourhood (16+2*pr fine cells per axis) are staged in LDS, bucketed
+ * per fine cell in chain order (decreasing l, the order of the reference's
+ * hoc/ll lists), as exact fixed-point positions X = x * 2^15 in the tile
+ * frame (x = ncell*(cell-1) + ncell*offset needs at most 10+15 bits).
+ * x2 - x1 = (X2 - X1) * 2^-15 is then the same FP64 value the reference
+ * forms, and each particle sums its pairs in the same order, so the result
+ * is bit-identical to the unstaged walk.  A block whose neighbourhood holds
+ * more than `maxs` particles walks the global lists instead.
+ */
+#define PPB_T 256
+__device__ static inline int pp_fix(int IX, int cc, int32_t code) {
+  return (cc - 1) * (4 * 32768) + (int)(2 * cube_ucode(IX, code) + 1) * (IX == 1 ? 256 : 1);
+}
+
+template <int IX, int IV>
+__global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *__restrict__ xp,
+                                                  typename IntOf<IV>::T *__restrict__ vp,
+                                                  const int32_t *__restrict__ rhoc,
+                                                  const int64_t *__restrict__ cstart,
+                                                  const double *__restrict__ vdec, Geo g, int nvh, int pr,
+                                                  double rsoft, double mass, double a_mid, double dt, double venc,
+                                                  int maxs, unsigned long long *__restrict__ out) {
+  extern __shared__ int smem[];
+  const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
+  int *cur = smem;             /* per fine cell: count, then offset, then end */
+  int *pos = smem + nreg + 1;  /* staged positions, 3 ints each */
+  __shared__ int part[PPB_T];
+  __shared__ int64_t cst[16][5];
+  __shared__ int rowoff[17];
+  const int tid = threadIdx.x, nt = g.nt, nb = nt / 4, nnt = g.nnt;
+  const int sh = IX == 1 ? 6 : 14;
+  int64_t b = blockIdx.x;
+  const int bx = (int)(b % nb), by = (int)((b / nb) % nb), bz = (int)((b / ((int64_t)nb * nb)) % nb);
+  const int t = (int)(b / ((int64_t)nb * nb * nb));
+  const int tt[3] = {t % nnt, (t / nnt) % nnt, t / (nnt * nnt)};
+  const int B0[3] = {4 * bx, 4 * by, 4 * bz};             /* cells B0+1..B0+4 (1-based, tile frame) */
+  const int flo[3] = {4 * B0[0] + 1 - pr, 4 * B0[1] + 1 - pr, 4 * B0[2] + 1 - pr};
+
+  for (int i = tid; i < nreg; i += PPB_T) cur[i] = 0;
+  if (tid < 16) {
+    const int ry = tid % 4, rz = tid / 4;
+    const int64_t c0 = g.cid(B0[0], B0[1] + ry, B0[2] + rz, t);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) cst[tid][q] = cstart[c0 + q];
+  }
+  __syncthreads();
+  /* count: one thread per staged coarse cell B0..B0+5 per axis */
+  const bool stager = tid < 216;
+  int scc[3] = {0, 0, 0}, snp = 0;
+  int64_t sq0 = 0;
+  if (stager) {
+    scc[0] = B0[0] + tid % 6;
+    scc[1] = B0[1] + (tid / 6) % 6;
+    scc[2] = B0[2] + tid / 36;
+    const int64_t cq = cid_global(g, tt[0] * nt + scc[0] - 1, tt[1] * nt + scc[1] - 1, tt[2] * nt + scc[2] - 1);
+    snp = rhoc[cq];
+    sq0 = cstart[cq];
+    for (int l = 0; l < snp; ++l) {
+      int r[3];
+      bool in = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int f = 4 * (scc[a] - 1) + (int)(cube_ucode(IX, (int32_t)xp[3 * (sq0 + l) + a]) >> sh) + 1;
+        r[a] = f - flo[a];
+        in = in && r[a] >= 0 && r[a] < RF;
+      }
+      if (in) cur[(r[2] * RF + r[1]) * RF + r[0]] += 1;
+    }
+  }
+  __syncthreads();
+  /* exclusive scan of the counts */
+  const int chunk = (nreg + PPB_T - 1) / PPB_T;
+  const int i0 = tid * chunk, i1 = min(i0 + chunk, nreg);
+  int s = 0;
+  for (int i = i0; i < i1; ++i) s += cur[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int o = 1; o < PPB_T; o <<= 1) {
+    const int v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  const int nstage = part[PPB_T - 1];
+  s = part[tid] - s;
+  for (int i = i0; i < i1; ++i) {
+    const int cnt = cur[i];
+    cur[i] = s;
+    s += cnt;
+  }
+  __syncthreads();
+  const bool staged = nstage <= maxs; /* uniform over the block */
+  if (staged && stager) {
+    for (int l = snp - 1; l >= 0; --l) {
+      int r[3], X[3];
+      bool in = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int32_t code = (int32_t)xp[3 * (sq0 + l) + a];
+        const int f = 4 * (scc[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
+        r[a] = f - flo[a];
+        in = in && r[a] >= 0 && r[a] < RF;
+        X[a] = pp_fix(IX, scc[a], code);
+      }
+      if (in) {
+        const int slot = cur[(r[2] * RF + r[1]) * RF + r[0]]++;
+        pos[3 * slot] = X[0];
+        pos[3 * slot + 1] = X[1];
+        pos[3 * slot + 2] = X[2];
+      }
+    }
+  }
+  if (tid == 0) {
+    int acc = 0;
+    for (int q = 0; q < 16; ++q) {
+      rowoff[q] = acc;
+      acc += (int)(cst[q][4] - cst[q][0]);
+    }
+    rowoff[16] = acc;
+  }
+  __syncthreads();
+  /* cur[f] now holds the end of fine cell f's list; its start is cur[f-1] */
+
+  double f2m = 0;
+  unsigned long long ntest = 0;
+  const int ntot = rowoff[16];
+  for (int i = tid; i < ntot; i += PPB_T) {
+    int row = 0;
+    while (rowoff[row + 1] <= i) ++row;
+    const int64_t p = cst[row][0] + (i - rowoff[row]);
+    int cx = 0;
+    while (cx < 3 && cst[row][cx + 1] <= p) ++cx;
+    const int loc[3] = {B0[0] + 1 + cx, B0[1] + 1 + row % 4, B0[2] + 1 + row / 4};
+    double x1[3], vreal[3], ft[3] = {0, 0, 0};
+    int X1[3], gf[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int32_t code = (int32_t)xp[3 * p + a];
+      x1[a] = (double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, code);
+      X1[a] = pp_fix(IX, loc[a], code);
+      gf[a] = 4 * (loc[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
+      vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
+    }
+    for (int kk = gf[2] - pr; kk <= gf[2] + pr; ++kk)
+      for (int jj = gf[1] - pr; jj <= gf[1] + pr; ++jj)
+        for (int ii = gf[0] - pr; ii <= gf[0] + pr; ++ii) {
+          if (staged) {
+            const int f = ((kk - flo[2]) * RF + (jj - flo[1])) * RF + (ii - flo[0]);
+            const int e = cur[f], s0 = f > 0 ? cur[f - 1] : 0;
+            for (int q = s0; q < e; ++q) {
+              double x21[3];
+#pragma unroll
+              for (int a = 0; a < 3; ++a) x21[a] = (double)(pos[3 * q + a] - X1[a]) * (1.0 / 32768.0);
+              double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
+              if (rmag == 0.0) rmag = 1.0;
+              const double rcut = rmag / 16.0;
+              const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
+              const double r3 = powi3(rmag);
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                double fp = (mass * (x21[a] / r3)) * pcut;
+                if (!(rmag > rsoft)) fp = fp * 0.0;
+                ft[a] = ft[a] + fp;
+              }
+            }
+          } else {
+            const int fc[3] = {ii, jj, kk};
+            int cc[3], sub[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              cc[a] = (int)floorf((float)(fc[a] - 1) / 4.0f) + 1;
+              sub[a] = (fc[a] - 1) - 4 * (cc[a] - 1);
+            }
+            const int64_t cq =
+                cid_global(g, tt[0] * nt + cc[0] - 1, tt[1] * nt + cc[1] - 1, tt[2] * nt + cc[2] - 1);
+            const int np = rhoc[cq];
+            const int64_t q0 = cstart[cq];
+            for (int l = np - 1; l >= 0; --l) {
+              const int64_t q = q0 + l;
+              const int32_t cd[3] = {(int32_t)xp[3 * q], (int32_t)xp[3 * q + 1], (int32_t)xp[3 * q + 2]};
+              if ((int)(cube_ucode(IX, cd[0]) >> sh) != sub[0] || (int)(cube_ucode(IX, cd[1]) >> sh) != sub[1] ||
+                  (int)(cube_ucode(IX, cd[2]) >> sh) != sub[2])
+                continue;
+              double x21[3];
+#pragma unroll
+              for (int a = 0; a < 3; ++a)
+                x21[a] = ((double)(4 * (cc[a] - 1)) + 4.0 * cube_xoff(IX, cd[a])) - x1[a];
+              double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
+              if (rmag == 0.0) rmag = 1.0;
+              const double rcut = rmag / 16.0;
+              const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
+              const double r3 = powi3(rmag);
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                double fp = (mass * (x21[a] / r3)) * pcut;
+                if (!(rmag > rsoft)) fp = fp * 0.0;
+                ft[a] = ft[a] + fp;
+              }
+            }
+          }
+        }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      vreal[a] = vreal[a] + (((ft[a] * a_mid) * dt) / 6.0) / (double)CUBE_PI_F;
+      vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal[a]);
+    }
+    const double f2 = (ft[0] * ft[0] + ft[1] * ft[1]) + ft[2] * ft[2];
+    f2m = f2 > f2m ? f2 : f2m;
+    ++ntest;
+  }
+  /* block reduction through the (now free) staging area */
+  __syncthreads();
+  double *red = reinterpret_cast<double *>(smem);
+  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(smem) + PPB_T;
+  red[tid] = f2m;
+  cnt[tid] = ntest;
+  __syncthreads();
+  for (int o = PPB_T / 2; o > 0; o >>= 1) {
+    if (tid < o) {
+      red[tid] = fmax(red[tid], red[tid + o]);
+      cnt[tid] += cnt[tid + o];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+    atomicAdd(out + 1, cnt[0]);
+  }
+}
+
 __global__ void k_zero_u64_pp(unsigned long long *p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
 }
 
 template <int IX, int IV>
 static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, unsigned long long *out) {
+  const int pr = (int)c->p.pp_range;
+  if (c->nt % 4 == 0 && pr >= 0 && pr <= 4) {
+    /* 64 KB of LDS per workgroup: the fine-cell table, then the staging area */
+    const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
+    const int dyn_ints = (65536 - 2048) / 4;
+    int maxs = (dyn_ints - (nreg + 1)) / 3;
+    /* a lower cap sends more workgroups down the unstaged walk (used by the
+       tests to cover both walks in one run) */
+    if (const char *e = getenv("CUBE_PP_STAGE_MAX")) maxs = std::min(maxs, atoi(e));
+    const int64_t nb = c->nt / 4, nblk = nb * nb * nb * c->ntiles;
+    hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3((unsigned)nblk), dim3(PPB_T), (size_t)dyn_ints * 4, c->stream,
+                       (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
+                       c->vdec0, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p, (double)a_mid,
+                       (double)dt, cube_vconst(sigma_vi), maxs, out);
+    CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
   const int64_t nrows = c->ncells / c->nt;
   hipLaunchKernelGGL((k_pp<IX, IV>), dim3((unsigned)nrows), dim3(PT), 0, c->stream,
                      (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,

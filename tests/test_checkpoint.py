@@ -1,0 +1,50 @@
+"""Checkpoint files in the reference format (W/checkpoint.f90:12-55,
+W/particle_initialization.f90:479-509): header layout, file names, widths."""
+import numpy as np
+import pytest
+
+from cafproject_amd import checkpoint as ck
+
+
+def test_header_is_the_reference_record():
+    # 17 integer(8) then 22 real(4) (W/parameters.f90:136-160)
+    assert ck.HEADER_DTYPE.itemsize == 17 * 8 + 22 * 4
+    assert ck.INT_FIELDS[:2] == ["nplocal", "npglobal"] and ck.REAL_FIELDS[0] == "a"
+    assert ck.HEADER_DTYPE.fields["a"][1] == 17 * 8
+
+
+@pytest.mark.parametrize("z,s", [(49.0, "49.000"), (0.0, "0.000"), (1.5, "1.500"), (127.25, "127.250")])
+def test_redshift_label_is_f7_3_left_adjusted(z, s):
+    assert ck.zstr(z) == s
+
+
+@pytest.mark.parametrize("izipx,izipv", [(1, 2), (2, 1), (2, 2)])
+def test_write_read_roundtrip_with_file_widths(tmp_path, izipx, izipv):
+    rng = np.random.default_rng(3)
+    n, nc = 1000, 8
+    lo = {1: -128, 2: -32768}
+    hi = {1: 127, 2: 32767}
+    # callers may hold the codes in a wider dtype; the files use izip bytes
+    xp = rng.integers(lo[izipx], hi[izipx] + 1, (n, 3)).astype(np.int32)
+    vp = rng.integers(lo[izipv], hi[izipv] + 1, (n, 3)).astype(np.int64)
+    rhoc = rng.integers(0, 5, nc ** 3).astype(np.int32)
+    vf = rng.standard_normal((nc ** 3, 3)).astype(np.float32)
+    pid = np.arange(1, n + 1, dtype=np.int32)
+    h = ck.new_header(npglobal=n, izipx=izipx, izipv=izipv, nnt=1, nt=nc, a=0.02, sigma_vi=0.5)
+    fn = ck.write(str(tmp_path), 49.0, h, xp, vp, rhoc, vf, pid)
+    assert fn["xp"].endswith("image1/49.000_xp_1.bin")
+    assert (tmp_path / "image1" / "49.000_xp_1.bin").stat().st_size == 3 * n * izipx
+    assert (tmp_path / "image1" / "49.000_vp_1.bin").stat().st_size == 3 * n * izipv
+    hr, st = ck.read(str(tmp_path), 49.0, nc)
+    assert int(hr["nplocal"]) == n and np.float32(hr["a"]) == np.float32(0.02)
+    assert np.array_equal(st["xp"], xp) and np.array_equal(st["vp"], vp)
+    assert np.array_equal(st["rhoc"], rhoc) and np.array_equal(st["vfield"], vf)
+    assert np.array_equal(st["pid"], pid)
+
+
+def test_write_refuses_codes_wider_than_the_header(tmp_path):
+    h = ck.new_header(izipx=1, izipv=2, nnt=1, nt=2)
+    xp = np.full((4, 3), 300, np.int16)
+    with pytest.raises(ValueError):
+        ck.write(str(tmp_path), 0.0, h, xp, np.zeros((4, 3), np.int16), np.zeros(8, np.int32),
+                 np.zeros((8, 3), np.float32))

@@ -41,11 +41,20 @@ def test_fortran_driver_matches_ctypes_replay(tmp_path, pp):
                       sigma_vi=st["sigma_vi"], z_i=49.0, vz_max=vz0)
     z_in = 1 / st["a"] - 1
     ck.write(str(tmp_path / "in"), z_in, h, st["xp"], st["vp"], st["rhoc"], st["vfield"], st["pid"])
+
+    # zero steps: checkpoint in -> GPU state -> checkpoint out is the identity
+    r = subprocess.run([DRIVER, str(tmp_path / "in"), "%.6f" % z_in, str(tmp_path / "out0"), "0.0", "0",
+                        "2", str(pp), DATA], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, out0 = ck.read(str(tmp_path / "out0"), 0.0, nc)
+    for k in ("xp", "vp", "rhoc", "vfield", "pid"):
+        assert np.array_equal(np.asarray(out0[k]).reshape(-1), np.asarray(st[k]).reshape(-1)), k
+
     r = subprocess.run([DRIVER, str(tmp_path / "in"), "%.6f" % z_in, str(tmp_path / "out"), "0.0", str(nsteps),
                         "2", str(pp), DATA], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     steps = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("step")]
-    assert len(steps) == nsteps
+    assert len(steps) == nsteps, r.stdout
 
     hdr, out = ck.read(str(tmp_path / "out"), 0.0, nc)
     assert int(hdr["timestep"]) == nsteps
@@ -72,7 +81,7 @@ def test_fortran_driver_matches_ctypes_replay(tmp_path, pp):
         pm = g.particle_mesh(a_mid, dt, float(sigma), vs["sigma_vi_new"], st["mass_p"])
         g.buffer_v()
         sigma = np.float32(vs["sigma_vi_new"])
-        assert sigma == np.float32(sig_new)
+        assert sigma == np.float32(sig_new), (r.stdout, vs, float(vz), dt_old, dt)
         vz = np.float32(pm["vz_max"])
         dtc.update(dt_fine=pm["dt_fine"], dt_coarse=pm["dt_coarse"], dt_vmax=pm["dt_vmax"])
     mine = g.get_state()

@@ -143,7 +143,13 @@ def test_kicks_bit_exact_given_forces(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_ids)
-def test_pp_bit_exact(case):
+@pytest.mark.parametrize("stage_max", [None, "0", "40"])
+def test_pp_bit_exact(case, stage_max, monkeypatch):
+    """PP stages each 4^3-cell block's neighbourhood in LDS; blocks with more
+    than CUBE_PP_STAGE_MAX particles there walk the global lists.  Both walks
+    (all staged, none staged, mixed) must match the oracle bit for bit."""
+    if stage_max is not None:
+        monkeypatch.setenv("CUBE_PP_STAGE_MAX", stage_max)
     st, oc, g = _pair(case, pp_on=True)
     m = st["mass_p"]
     f_o = oc.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
