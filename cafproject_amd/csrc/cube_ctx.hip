@@ -119,7 +119,7 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->xkey, c->xcap);
   st |= dalloc(&c->rank, c->cap);
   st |= dalloc(&c->xrank, c->xcap);
-  st |= dalloc(&c->fbflag, c->ncells / 128 + 2);
+  st |= dalloc(&c->fbflag, c->ncells / 64 + 2); /* one per 64-cell finalize workgroup of the drift */
   st |= dalloc(&c->tanf_tab, c->nvbin);
   st |= dalloc(&c->vdec0, c->nvbin);
   st |= dalloc(&c->vdec1, c->nvbin);
@@ -140,7 +140,7 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->ured, 64);
   st |= dalloc(&c->ired, 4096);
   st |= dalloc(&c->scanbuf, c->ncells / 1024 + 4);
-  st |= dalloc(&c->vpart, 3 * (c->ncells / 128 + 2));
+  st |= dalloc(&c->vpart, 3 * (c->ncells / 64 + 2));
   if (st) {
     free_all(c);
     delete c;

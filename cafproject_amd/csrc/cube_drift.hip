@@ -28,6 +28,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 
 #define DT 256
 static const double WEIGHT_V = (double)0.1f; /* real(8),parameter :: weight_v=0.1 */
@@ -176,7 +177,9 @@ struct DriftP {
   int nvh;        /* nvbin/2 */
   float dt_mid;
   double xres;
+  int agg_cap;    /* largest row (particles) whose slots are aggregated in LDS */
 };
+#define AGG_CAP 4096
 
 template <int IX, int IV>
 __global__ __launch_bounds__(DT) void k_drift_dest(const typename IntOf<IX>::T *__restrict__ xp,
@@ -196,11 +199,23 @@ __global__ __launch_bounds__(DT) void k_drift_dest(const typename IntOf<IX>::T *
   const int j = (int)(row % nt), k = (int)((row / nt) % nt), t = (int)(row / ((int64_t)nt * nt));
   const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
   __shared__ int64_t cs[520];
+  /* Slots inside destination cells: a main copy landing in the 3x3 rows of
+     cells around this row (same tile) takes a local rank from an LDS
+     counter, and each such cell then reserves its range with one global
+     atomic; other copies take their slot with a global atomic directly. */
+  extern __shared__ int dyn[];
+  const int nwin = 9 * nt;
+  int *lcnt = dyn;               /* nwin counters, then the reserved bases */
+  /* per particle of the row: window cell << 19 | local rank, or ~0u */
+  uint32_t *pslot = reinterpret_cast<uint32_t *>(dyn + nwin);
   const int64_t c0 = row * nt;
   for (int i = threadIdx.x; i <= nt; i += DT) cs[i] = cstart[c0 + i];
+  for (int i = threadIdx.x; i < nwin; i += DT) lcnt[i] = 0;
   __syncthreads();
   const int64_t pbeg = cs[0], pend = cs[nt];
+  const bool agg = pend - pbeg <= P.agg_cap;   /* uniform over the block */
   for (int64_t p = pbeg + threadIdx.x; p < pend; p += DT) {
+    if (agg) pslot[p - pbeg] = ~0u;
     /* cell of p: largest i with cs[i] <= p */
     int lo = 0, hi = nt;
     while (hi - lo > 1) {
@@ -258,25 +273,47 @@ __global__ __launch_bounds__(DT) void k_drift_dest(const typename IntOf<IX>::T *
                                             dt_[0] + g.nnt * (dt_[1] + g.nnt * dt_[2]));
           const uint64_t kk = ((uint64_t)il << 55) | ((uint64_t)(r[2] & 31) << 50) |
                               ((uint64_t)(r[1] & 31) << 45) | ((uint64_t)(r[0] & 31) << 40) | (uint64_t)p;
-          /* the count's old value is a free slot in the cell (arrival
-             order; the per-cell sort restores the reference order) */
-          const int32_t rk = atomicAdd(cnt + dc, 1);
+          /* slots are taken in arrival order; the per-cell sort restores
+             the reference order */
           if (w == 0) {
             dest[p] = dc;
             key[p] = kk;
-            rank[p] = rk;
+            const int ddy = lig[1][qy] - 1 - j, ddz = lig[2][qz] - 1 - k;
+            if (agg && dt_[0] == tx && dt_[1] == ty && dt_[2] == tz && ddy >= -1 && ddy <= 1 && ddz >= -1 &&
+                ddz <= 1) {
+              const int wi = ((ddz + 1) * 3 + (ddy + 1)) * nt + (lig[0][qx] - 1);
+              const int lr = atomicAdd(lcnt + wi, 1);
+              pslot[p - pbeg] = ((uint32_t)wi << 19) | (uint32_t)lr;
+            } else {
+              rank[p] = atomicAdd(cnt + dc, 1);
+            }
           } else {
             const unsigned long long x = atomicAdd(counters + 0, 1ull);
             if ((int64_t)x < xcap) {
               xdest[x] = dc;
               xkey[x] = kk;
-              xrank[x] = rk;
+              xrank[x] = atomicAdd(cnt + dc, 1);
             } else {
               err |= DERR_XOVER;
             }
           }
         }
     if (err) atomicOr(counters + 2, (unsigned long long)err);
+  }
+  if (!agg) return; /* uniform */
+  __syncthreads();
+  /* reserve each window cell's range: one global atomic per cell */
+  for (int wi = threadIdx.x; wi < nwin; wi += DT) {
+    const int n = lcnt[wi];
+    if (n > 0) {
+      const int y = j + (wi / nt) % 3 - 1, z = k + (wi / nt) / 3 - 1;
+      lcnt[wi] = atomicAdd(cnt + g.cid(wi % nt, y, z, t), n);
+    }
+  }
+  __syncthreads();
+  for (int64_t p = pbeg + threadIdx.x; p < pend; p += DT) {
+    const uint32_t s = pslot[p - pbeg];
+    if (s != ~0u) rank[p] = lcnt[s >> 19] + (int)(s & 0x7ffffu);
   }
 }
 
@@ -305,8 +342,8 @@ __global__ void k_place_ranked(const int32_t *__restrict__ dest, const int32_t *
 /* ---------------- per-cell sort ---------------- */
 #define SMALL_SORT 48
 #define BIG_SORT 16384
-#define FB_CELLS 128     /* cells per finalize workgroup */
-#define FB_CH 2048       /* slots staged in LDS per finalize pass */
+#define FB_CELLS 64      /* cells per finalize workgroup (the context sizes fbflag/vpart for it) */
+#define FB_CH 1024       /* slots staged in LDS per finalize pass */
 #define FB_LDS_SORT 64   /* largest cell sorted inside the finalize workgroup */
 /* flag[b] = 1 when finalize block b (cells b*FB_CELLS...) holds at most FB_CH
    particles and no cell above FB_LDS_SORT: it sorts its cells in LDS */
@@ -463,15 +500,23 @@ __global__ __launch_bounds__(256) void k_drift_final2(
     double venc, typename IntOf<IX>::T *__restrict__ xp2, typename IntOf<IV>::T *__restrict__ vp2,
     int32_t *__restrict__ pid2, float *__restrict__ vfield_new, int64_t ncells, double *__restrict__ part) {
   __shared__ int64_t cst[FB_CELLS + 1];
-  __shared__ double vr[FB_CH][3];
+  __shared__ double vr[FB_CH][3]; /* decoded velocities; the final reduction reuses it */
   __shared__ uint64_t pix[FB_CH]; /* sorted keys, then source particle indices */
+  __shared__ uint8_t lcell[FB_CH]; /* slot -> local cell */
+  __shared__ int cgeo[FB_CELLS][4]; /* local i, j, k and tile of each cell */
   __shared__ float vn[FB_CELLS][3];
-  __shared__ double red[3][256];
   const Geo g = P.g;
   const int nt = g.nt, ncb = g.ncb, tid = threadIdx.x;
   const int64_t c0 = (int64_t)blockIdx.x * FB_CELLS;
   const int ncl = (int)min((int64_t)FB_CELLS, ncells - c0);
   for (int i = tid; i <= ncl; i += 256) cst[i] = start[c0 + i];
+  if (tid < ncl) {
+    const int64_t cidx = c0 + tid;
+    cgeo[tid][0] = (int)(cidx % nt);
+    cgeo[tid][1] = (int)((cidx / nt) % nt);
+    cgeo[tid][2] = (int)((cidx / ((int64_t)nt * nt)) % nt);
+    cgeo[tid][3] = (int)(cidx / ((int64_t)nt * nt * nt));
+  }
   __syncthreads();
   const int64_t S0 = cst[0], S1 = cst[ncl];
   /* every slot fits (and every cell is small): sort the keys of each cell
@@ -483,6 +528,7 @@ __global__ __launch_bounds__(256) void k_drift_final2(
     if (tid < ncl) {
       /* insertion sort of the cell's keys (unique: they carry the particle index) */
       const int lo = (int)(cst[tid] - S0), hi = (int)(cst[tid + 1] - S0);
+      for (int q = lo; q < hi; ++q) lcell[q] = (uint8_t)tid;
       for (int q = lo + 1; q < hi; ++q) {
         const uint64_t x = pix[q];
         int r = q - 1;
@@ -495,25 +541,32 @@ __global__ __launch_bounds__(256) void k_drift_final2(
     }
     __syncthreads();
   }
-
-  /* slot -> (local cell, source particle, decoded velocity) */
-  auto gather = [&](int64_t s, int &lc, int64_t &p, double v[3]) {
+  auto cell_of = [&](int64_t s) -> int {
+    if (one) return lcell[s - S0];
     int lo = 0, hi = ncl;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
       if (cst[mid] <= s) lo = mid; else hi = mid;
     }
-    lc = lo;
-    const int64_t cidx = c0 + lo;
-    const int i = (int)(cidx % nt), j = (int)((cidx / nt) % nt), k = (int)((cidx / ((int64_t)nt * nt)) % nt);
-    const int t = (int)(cidx / ((int64_t)nt * nt * nt));
-    const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
+    return lo;
+  };
+
+  /* slot -> (local cell, source particle, decoded velocity) */
+  auto gather = [&](int64_t s, int &lc, int64_t &p, double v[3]) {
+    lc = cell_of(s);
+    const int i = cgeo[lc][0], j = cgeo[lc][1], k = cgeo[lc][2], t = cgeo[lc][3];
     const uint64_t kk = one ? pix[s - S0] : keys[s];
     p = (int64_t)(kk & KEY_PMASK);
     const int rx = (int)((kk >> 40) & 31), ry = (int)((kk >> 45) & 31), rz = (int)((kk >> 50) & 31);
     /* source cell = destination + (r - 2ncb), in the destination tile frame */
-    const int64_t sc = cid_global(g, tx * nt + i + rx - 2 * ncb, ty * nt + j + ry - 2 * ncb,
-                                  tz * nt + k + rz - 2 * ncb);
+    const int sx = i + rx - 2 * ncb, sy = j + ry - 2 * ncb, sz = k + rz - 2 * ncb;
+    int64_t sc;
+    if (sx >= 0 && sx < nt && sy >= 0 && sy < nt && sz >= 0 && sz < nt) {
+      sc = g.cid(sx, sy, sz, t);
+    } else {
+      const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
+      sc = cid_global(g, tx * nt + sx, ty * nt + sy, tz * nt + sz);
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a] = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vfield_old[3 * sc + a];
   };
@@ -562,12 +615,7 @@ __global__ __launch_bounds__(256) void k_drift_final2(
     int64_t p;
     double v[3];
     if (one) {
-      int lo = 0, hi = ncl;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (cst[mid] <= s) lo = mid; else hi = mid;
-      }
-      lc = lo;
+      lc = lcell[s - S0];
       p = (int64_t)pix[s - S0];
       v[0] = vr[s - S0][0];
       v[1] = vr[s - S0][1];
@@ -593,6 +641,8 @@ __global__ __launch_bounds__(256) void k_drift_final2(
     r[2] += (double)vn[lc][2];
     ss += (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
   }
+  __syncthreads(); /* vr is free: reduce through it */
+  double(*red)[256] = reinterpret_cast<double(*)[256]>(&vr[0][0]);
   red[0][tid] = sc;
   red[1][tid] = sr;
   red[2][tid] = ss;
@@ -735,7 +785,8 @@ template <int IX, int IV> static int drift_impl(cube_ctx *c, DriftP P, double ve
   if ((st = fill_i32(c, c->rhoc2, c->ncells, 0))) return st;
   if ((st = fill_i64(c, (int64_t *)ctr, 8, 0))) return st;
   const int64_t nrows = c->ncells / c->nt;
-  hipLaunchKernelGGL((k_drift_dest<IX, IV>), dim3((unsigned)nrows), dim3(DT), 0, c->stream, (const TX *)c->xp,
+  const size_t dyn = (size_t)(9 * c->nt + P.agg_cap) * sizeof(int);
+  hipLaunchKernelGGL((k_drift_dest<IX, IV>), dim3((unsigned)nrows), dim3(DT), dyn, c->stream, (const TX *)c->xp,
                      (const TV *)c->vp, c->vfield, c->cstart, c->vdec0, P, c->dest, c->key, c->rhoc2, c->rank,
                      c->xdest, c->xkey, c->xrank, c->xcap, ctr);
   CUBE_LAUNCH_CHECK();
@@ -812,6 +863,10 @@ int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi
   P.nvh = c->nvbin / 2;
   P.dt_mid = dt_mid;
   P.xres = c->p.izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0;
+  P.agg_cap = AGG_CAP;
+  /* a lower cap sends more rows down the all-global slot path (used by the
+     tests to cover both paths in one run) */
+  if (const char *e = getenv("CUBE_DRIFT_AGG_CAP")) P.agg_cap = std::min(AGG_CAP, atoi(e));
   const double venc = cube_vconst(sigma_vi);
   int64_t nxtra = 0;
   unsigned long long hc[8];

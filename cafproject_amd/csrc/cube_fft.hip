@@ -601,7 +601,10 @@ template <int N>
 __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ specd, float *__restrict__ force,
                                                 const float2 *__restrict__ tw, int64_t nrows, float div,
                                                 unsigned int *__restrict__ f2max) {
-  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC, TL = FFT_T / B, LP = L + 1;
+  /* half the rows of the single-component kernels: three components in LDS
+     at 2 rows per workgroup keep 6 workgroups per CU, which measures faster
+     than 4 rows at 3 workgroups per CU despite idle threads in the passes */
+  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = L + 1;
   __shared__ float2 sm[3 * B * LP];
   __shared__ float red[FFT_T];
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
@@ -772,7 +775,7 @@ template <int N> struct Launch {
     return CUBE_OK;
   }
   static int c2r3(cube_ctx *c, FFTPlan *p, const float2 *work3, float *force, float div, unsigned *f2) {
-    constexpr int B = Cfg<N>::BC;
+    constexpr int BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC;
     const int64_t rows = (int64_t)N * N;
     hipLaunchKernelGGL(k_c2r_x3<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0, c->stream, work3,
                        force, p->tw, rows, div, f2);

@@ -55,7 +55,13 @@ def test_update_x_bit_exact(case, dt):
 
 
 @pytest.mark.parametrize("case", CASES[:2], ids=_ids)
-def test_two_drifts_bit_exact(case):
+@pytest.mark.parametrize("agg_cap", [None, "0", "300"])
+def test_two_drifts_bit_exact(case, agg_cap, monkeypatch):
+    """The drift takes slots in new cells through LDS counters per source row
+    (rows up to CUBE_DRIFT_AGG_CAP particles) or global atomics; all-LDS,
+    all-global and mixed runs must match the oracle bit for bit."""
+    if agg_cap is not None:
+        monkeypatch.setenv("CUBE_DRIFT_AGG_CAP", agg_cap)
     st, oc, g = _pair(case)
     vz = float(np.abs(st["vfield"][:, 2]).max() * 1.5 + 0.02)
     sig = st["sigma_vi"]
