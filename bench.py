@@ -163,6 +163,43 @@ def cpu_baseline(seconds_cap=60.0):
                 seconds=t)
 
 
+def batched_gradients(iters=3):
+    """The second half of the metric: forward (10 KDK steps) + MSE loss +
+    adjoint gradients of an uneven ensemble of 48 differentiable-PM boxes
+    (16x32^3, 16x64^3, 8x96^3, 8x128^3 particles, fine mesh 2x the particle
+    grid, 50-200 Mpc/h, sigma8 in [0.7, 0.9], seeds 1000-1047), as one
+    replayed HIP graph against the per-box loop of the same kernels on one
+    stream.  grads/s = boxes / wall time of one ensemble pass."""
+    import torch
+    from cafproject_amd import dpm
+    t = time.time()
+    b, tens = dpm.make_ensemble()
+    nbox = len(tens)
+    shapes = b.tune()
+    log(f"[bench] ensemble of {nbox} boxes built and tuned in {time.time() - t:.1f}s")
+    res = {}
+    for name, graph in (("per_box_loop", False), ("graph", True)):
+        b.run(graph=graph)  # warm-up (and capture)
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            b.run(graph=graph)
+        b.sync()
+        dt = (time.perf_counter() - t0) / iters
+        res[name] = dict(ms_per_pass=1e3 * dt, grads_per_s=nbox / dt)
+        log(f"[bench] ensemble {name}: {1e3 * dt:.1f} ms per pass, {nbox / dt:.1f} grads/s")
+    torch.cuda.synchronize()
+    loss = [float(x["loss"].item()) for x in tens]
+    b.close()
+    return {"metric": "batched adjoint grads/s (forward + adjoint gradient per box)", "boxes": nbox,
+            "workload": "48 boxes: 16x32^3, 16x64^3, 8x96^3, 8x128^3 particles, fine mesh 2x particle grid, "
+                        "10 KDK steps a=0.1->1, MSE loss, per-step checkpoints",
+            "graph_grads_per_s": res["graph"]["grads_per_s"], "loop_grads_per_s": res["per_box_loop"]["grads_per_s"],
+            "speedup": res["graph"]["grads_per_s"] / res["per_box_loop"]["grads_per_s"], "iters": iters,
+            "detail": res, "tuned_shapes": sorted({(int(a), int(c)) for a, c in shapes}),
+            "loss_finite": bool(np.all(np.isfinite(loss)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +210,8 @@ def main():
     ap.add_argument("--box", type=float, default=400.0)
     ap.add_argument("--no-pp", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-gradient ensemble measurement")
+    ap.add_argument("--batched-iters", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy kernel (torch), the reference "
                          "point for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
@@ -256,6 +295,9 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "a_final": float(step.clock.a),
     }
+    if not args.no_batched:
+        del step, g
+        out["batched"] = batched_gradients(args.batched_iters)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline()

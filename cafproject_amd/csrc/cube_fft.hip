@@ -8,17 +8,19 @@
  * (e^{-i}), backward unnormalised (e^{+i}) followed by three divisions by N
  * in real(4) (W/pm.f90:123, W/pencil_fft.f90:58).
  *
- * Structure (mesh N^3, x fastest; half spectrum NH=N/2+1 along x):
+ * Structure (mesh N^3, x fastest; half spectrum of NH=N/2+1 complex per
+ * row, rows padded to cube_spec_pitch(N) so they start on cache lines):
  *   k_r2c_x   : rows along x, real N -> complex N/2 FFT + split -> NH
- *   k_c2c_s   : lines along y (stride NH) or z (stride N*NH), in place
- *   k_green_z : lines along z: forward FFT, then for d=0..2 multiply by
- *               i*K_d(k) and inverse FFT, writing three spectra
+ *   k_c2c_r / k_c2c_s : lines along y or z, in place (three spectra in one
+ *               launch for the inverse y pass)
+ *   k_green_r / k_green_z : lines along z: forward FFT, then for d=0..2
+ *               multiply by i*K_d(k) and inverse FFT, writing three spectra
  *   k_c2r_x3  : rows along x for the three spectra: merge -> complex N/2
  *               inverse FFT -> real, /N/N/N, write F as 3 floats per cell,
  *               and max |F|^2 (W/pm.f90:126,315)
- * Every line FFT is a Stockham autosort FFT in LDS (radix 8/4/2/3/5/7),
- * a batch of B lines per 256-thread workgroup; strided passes load B
- * consecutive x-columns per row so global accesses stay coalesced.
+ * Strided lines of power-of-two meshes 64..1024 (k_c2c_r, k_green_r) are
+ * register-resident four-step FFTs over 16-column batches; other sizes and
+ * the row passes use a Stockham autosort FFT in LDS (radix 8/4/2/3/5/7).
  */
 #include "cube_internal.h"
 #include <string.h>
@@ -859,6 +861,7 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
     default:
       return CUBE_ERR_INVALID;
   }
+  if (!f2max) return CUBE_OK; /* no readback: stays asynchronous (and capturable) */
   unsigned bits = 0;
   CUBE_CHECK(hipMemcpyAsync(&bits, c->ured, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
   CUBE_CHECK(hipStreamSynchronize(c->stream));

@@ -155,11 +155,14 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
                                                   int maxs, unsigned long long *__restrict__ out) {
   extern __shared__ int smem[];
   const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
-  int *cur = smem;             /* per fine cell: count, then offset, then end */
-  int *pos = smem + nreg + 1;  /* staged positions, 3 ints each */
+  int *cur = smem;                  /* per fine cell: count, then start offset (nreg+1) */
+  int *pos = smem + nreg + 1;       /* staged positions, 3 ints each (maxs) */
+  int16_t *sfi = reinterpret_cast<int16_t *>(pos + 3 * maxs); /* per walked particle: region cell or -1 */
   __shared__ int part[PPB_T];
   __shared__ int64_t cst[16][5];
   __shared__ int rowoff[17];
+  __shared__ int coff[217];         /* particles of the 216 staged cells: exclusive offsets */
+  __shared__ int64_t cq0[216];      /* their first particle */
   const int tid = threadIdx.x, nt = g.nt, nb = nt / 4, nnt = g.nnt;
   const int sh = IX == 1 ? 6 : 14;
   int64_t b = blockIdx.x;
@@ -176,28 +179,57 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
 #pragma unroll
     for (int q = 0; q < 5; ++q) cst[tid][q] = cstart[c0 + q];
   }
-  __syncthreads();
-  /* count: one thread per staged coarse cell B0..B0+5 per axis */
-  const bool stager = tid < 216;
-  int scc[3] = {0, 0, 0}, snp = 0;
-  int64_t sq0 = 0;
-  if (stager) {
-    scc[0] = B0[0] + tid % 6;
-    scc[1] = B0[1] + (tid / 6) % 6;
-    scc[2] = B0[2] + tid / 36;
-    const int64_t cq = cid_global(g, tt[0] * nt + scc[0] - 1, tt[1] * nt + scc[1] - 1, tt[2] * nt + scc[2] - 1);
+  /* the staged coarse cells B0..B0+5 per axis (tile frame, 1-based):
+     counts, first particles, and a scan of the counts */
+  int snp = 0;
+  if (tid < 216) {
+    const int sc[3] = {B0[0] + tid % 6, B0[1] + (tid / 6) % 6, B0[2] + tid / 36};
+    const int64_t cq = cid_global(g, tt[0] * nt + sc[0] - 1, tt[1] * nt + sc[1] - 1, tt[2] * nt + sc[2] - 1);
     snp = rhoc[cq];
-    sq0 = cstart[cq];
-    for (int l = 0; l < snp; ++l) {
-      int r[3];
-      bool in = true;
+    cq0[tid] = cstart[cq];
+  }
+  part[tid] = snp;
+  __syncthreads();
+  for (int o = 1; o < PPB_T; o <<= 1) {
+    const int v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  if (tid < 216) coff[tid + 1] = part[tid];
+  if (tid == 0) coff[0] = 0;
+  __syncthreads();
+  const int nwalk = coff[216];
+  const bool fits = nwalk <= 2 * maxs; /* sfi holds 2*maxs entries; uniform over the block */
+  /* the staged cell and sub-cell region index of walked particle j */
+  auto locate = [&](int j, int &q, int &l, int32_t code[3], int &ri) {
+    int lo = 0, hi = 216;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (coff[mid] <= j) lo = mid; else hi = mid;
+    }
+    q = lo;
+    l = j - coff[q];
+    const int sc[3] = {B0[0] + q % 6, B0[1] + (q / 6) % 6, B0[2] + q / 36};
+    int r[3];
+    bool in = true;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int f = 4 * (scc[a] - 1) + (int)(cube_ucode(IX, (int32_t)xp[3 * (sq0 + l) + a]) >> sh) + 1;
-        r[a] = f - flo[a];
-        in = in && r[a] >= 0 && r[a] < RF;
-      }
-      if (in) cur[(r[2] * RF + r[1]) * RF + r[0]] += 1;
+    for (int a = 0; a < 3; ++a) {
+      code[a] = (int32_t)xp[3 * (cq0[q] + l) + a];
+      const int f = 4 * (sc[a] - 1) + (int)(cube_ucode(IX, code[a]) >> sh) + 1;
+      r[a] = f - flo[a];
+      in = in && r[a] >= 0 && r[a] < RF;
+    }
+    ri = in ? (r[2] * RF + r[1]) * RF + r[0] : -1;
+  };
+  /* count per fine cell, all walked particles in parallel */
+  if (fits) {
+    for (int j = tid; j < nwalk; j += PPB_T) {
+      int q, l, ri;
+      int32_t code[3];
+      locate(j, q, l, code, ri);
+      sfi[j] = (int16_t)ri;
+      if (ri >= 0) atomicAdd(cur + ri, 1);
     }
   }
   __syncthreads();
@@ -221,26 +253,24 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     cur[i] = s;
     s += cnt;
   }
+  if (tid == 0) cur[nreg] = nstage;
   __syncthreads();
-  const bool staged = nstage <= maxs; /* uniform over the block */
-  if (staged && stager) {
-    for (int l = snp - 1; l >= 0; --l) {
-      int r[3], X[3];
-      bool in = true;
+  const bool staged = fits && nstage <= maxs; /* uniform over the block */
+  if (staged) {
+    /* chain order inside a fine cell is decreasing l: a particle's slot is
+       its cell's start plus the number of same-cell particles with larger l
+       (all in the same coarse cell) */
+    for (int j = tid; j < nwalk; j += PPB_T) {
+      if (sfi[j] < 0) continue;
+      int q, l, ri;
+      int32_t code[3];
+      locate(j, q, l, code, ri);
+      int rank = 0;
+      for (int j2 = j + 1; j2 < coff[q + 1]; ++j2) rank += sfi[j2] == ri;
+      const int slot = cur[ri] + rank;
+      const int sc[3] = {B0[0] + q % 6, B0[1] + (q / 6) % 6, B0[2] + q / 36};
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int32_t code = (int32_t)xp[3 * (sq0 + l) + a];
-        const int f = 4 * (scc[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
-        r[a] = f - flo[a];
-        in = in && r[a] >= 0 && r[a] < RF;
-        X[a] = pp_fix(IX, scc[a], code);
-      }
-      if (in) {
-        const int slot = cur[(r[2] * RF + r[1]) * RF + r[0]]++;
-        pos[3 * slot] = X[0];
-        pos[3 * slot + 1] = X[1];
-        pos[3 * slot + 2] = X[2];
-      }
+      for (int a = 0; a < 3; ++a) pos[3 * slot + a] = pp_fix(IX, sc[a], code[a]);
     }
   }
   if (tid == 0) {
@@ -252,7 +282,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     rowoff[16] = acc;
   }
   __syncthreads();
-  /* cur[f] now holds the end of fine cell f's list; its start is cur[f-1] */
+  /* fine cell f's list is pos[cur[f] .. cur[f+1]) */
 
   double f2m = 0;
   unsigned long long ntest = 0;
@@ -279,7 +309,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
         for (int ii = gf[0] - pr; ii <= gf[0] + pr; ++ii) {
           if (staged) {
             const int f = ((kk - flo[2]) * RF + (jj - flo[1])) * RF + (ii - flo[0]);
-            const int e = cur[f], s0 = f > 0 ? cur[f - 1] : 0;
+            const int s0 = cur[f], e = cur[f + 1];
             for (int q = s0; q < e; ++q) {
               double x21[3];
 #pragma unroll
@@ -369,10 +399,11 @@ template <int IX, int IV>
 static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, unsigned long long *out) {
   const int pr = (int)c->p.pp_range;
   if (c->nt % 4 == 0 && pr >= 0 && pr <= 4) {
-    /* 64 KB of LDS per workgroup: the fine-cell table, then the staging area */
+    /* 64 KB of LDS per workgroup: the fine-cell table, the staged positions
+       (3 ints each) and the walked particles' cell indices (2 per int) */
     const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
-    const int dyn_ints = (65536 - 2048) / 4;
-    int maxs = (dyn_ints - (nreg + 1)) / 3;
+    const int dyn_ints = (65536 - 4608) / 4;
+    int maxs = (dyn_ints - (nreg + 1)) / 4;
     /* a lower cap sends more workgroups down the unstaged walk (used by the
        tests to cover both walks in one run) */
     if (const char *e = getenv("CUBE_PP_STAGE_MAX")) maxs = std::min(maxs, atoi(e));

@@ -206,6 +206,16 @@ int cube_timers(cube_ctx *ctx, double *ms, int64_t *calls, int reset);
 /* Wait for all work of the context; returns CUBE_ERR_HIP on a device error. */
 int cube_sync(cube_ctx *ctx);
 
+/* Matter power spectrum of the current particle state, as the reference's
+   analysis computes it (WU/cicpower.f90:96-187, WU/powerspectrum.f90:28-135):
+   cell-centred CIC on an ng^3 mesh (ng divides 4*nc, ng <= 4*nc), delta =
+   rho/mean - 1, delta_k = FFT/ng^3, the kx=0 and kx=ng/2 planes' Hermitian
+   duplicates skipped, linear bins ibin = nint(|k|) (k in fundamental
+   units), Delta^2 = |delta_k|^2 / sinc^4 * 4 pi |k|^3.  Writes, for bins
+   1..nbin (nbin = nint(ng/2 * sqrt(3))), the per-bin sums of |k|, Delta^2
+   and the mode counts; the caller divides.  Uses the fine-mesh buffers. */
+int cube_power(cube_ctx *ctx, int32_t ng, int32_t nbin, double *ksum, double *psum, int64_t *count);
+
 /* Host-side twins of device math, exported for the bindings' checks. */
 double cube_host_atan(double x);
 

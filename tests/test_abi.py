@@ -11,10 +11,11 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HDR = os.path.join(ROOT, "include", "cube.h")
+HDRS = sorted(os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
 
 
 def declared():
-    src = open(HDR).read()
+    src = "".join(open(h).read() for h in HDRS)
     return sorted(set(re.findall(r"^\s*(?:int|void|double|const char \*)\s*\*?\s*(cube_\w+)\s*\(", src, re.M)))
 
 
@@ -26,6 +27,8 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert set(cube.EXPORTS) <= set(names)
+    from cafproject_amd import dpm
+    assert set(dpm.EXPORTS) <= set(names) and set(dpm.BATCH_EXPORTS) <= set(names)
 
 
 def test_strerror_without_gpu():
