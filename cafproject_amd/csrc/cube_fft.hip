@@ -609,6 +609,10 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
   constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = L + 1;
   __shared__ float2 sm[3 * B * LP];
   __shared__ float red[FFT_T];
+  /* a power-of-two divisor (every power-of-two mesh) has an exact inverse */
+  const uint32_t db = __float_as_uint(div);
+  const bool pow2 = (db & 0x807FFFFFu) == 0u && (db >> 23) > 0u && (db >> 23) < 200u;
+  const float inv = 1.0f / div;
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int64_t row0 = (int64_t)blockIdx.x * B;
   const int64_t plane = nrows * cube_spec_pitch(N);
@@ -640,9 +644,15 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
     for (int d = 0; d < 3; ++d) {
       const float2 z = sm[(d * B + l) * LP + (x >> 1)];
       float v = (x & 1) ? z.y : z.x;
-      v = v / div;
-      v = v / div;
-      v = v / div;
+      if (pow2) { /* exact: v/2^k and v*2^-k round the same real number */
+        v = v * inv;
+        v = v * inv;
+        v = v * inv;
+      } else {
+        v = v / div;
+        v = v / div;
+        v = v / div;
+      }
       f[d] = v;
     }
     if (row0 + l < nrows) {

@@ -204,23 +204,59 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
         T[a] = gg[a] / g.nt;
         C[a] = gg[a] - T[a] * g.nt + 1; /* 1-based local coarse cell */
       }
-      Src sz;
-      sz.n = 2; sz.c[0] = C[2] - 1; sz.c[1] = C[2];
-      order_z(sz, 2 - g.ncb);
+      /* the two source z cells in (ilayer, k) order, ilayer = (k-k0) mod 3 */
+      int za = C[2] - 1, zb = C[2];
+      {
+        const int k0 = 2 - g.ncb;
+        const int ka = ((za - k0) % 3 + 3) % 3, kb = ((zb - k0) % 3 + 3) % 3;
+        if (ka > kb) {
+          za = C[2];
+          zb = C[2] - 1;
+        }
+      }
+      /* counts and starts of the 8 source cells, all loads issued up front */
+      int npc[8];
+      int64_t p0c[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int64_t cc = cid_global(g, T[0] * g.nt + C[0] - 2 + (q & 1), T[1] * g.nt + C[1] - 2 + ((q >> 1) & 1),
+                                      T[2] * g.nt + ((q >> 2) ? zb : za) - 1);
+        npc[q] = rhoc[cc];
+        p0c[q] = cstart[cc];
+      }
+#pragma unroll 1
       for (int qz = 0; qz < 2; ++qz)
-        for (int sy = C[1] - 1; sy <= C[1]; ++sy)
-          for (int sx = C[0] - 1; sx <= C[0]; ++sx) {
-            const int ci[3] = {sx, sy, sz.c[qz]};
-            const int64_t cc = cid_global(g, T[0] * g.nt + ci[0] - 1, T[1] * g.nt + ci[1] - 1,
-                                          T[2] * g.nt + ci[2] - 1);
-            const int np = rhoc[cc];
-            const int64_t p0 = cstart[cc];
-            for (int l = 0; l < np; ++l) {
+#pragma unroll 1
+        for (int qy = 0; qy < 2; ++qy)
+#pragma unroll 1
+          for (int qx = 0; qx < 2; ++qx) {
+            const int ci[3] = {C[0] - 1 + qx, C[1] - 1 + qy, qz ? zb : za};
+            const int qi = (qz * 2 + qy) * 2 + qx;
+            int np = npc[0];
+            int64_t p0 = p0c[0];
+#pragma unroll
+            for (int q = 1; q < 8; ++q) { /* register select, no dynamic indexing */
+              np = qi == q ? npc[q] : np;
+              p0 = qi == q ? p0c[q] : p0;
+            }
+            /* four particles' codes are loaded together (the last one
+               repeated past the end) so their loads are in flight at once */
+            for (int l0 = 0; l0 < np; l0 += 4) {
+              int32_t code4[4][3];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int64_t q = p0 + min(l0 + u, np - 1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) code4[u][a] = (int32_t)xp[3 * q + a];
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+              if (l0 + u >= np) break;
               int r[3];
               float d1[3];
 #pragma unroll
               for (int a = 0; a < 3; ++a) {
-                const int32_t code = (int32_t)xp[3 * (p0 + l) + a];
+                const int32_t code = code4[u][a];
                 /* ncell*(i-1)+ncell*(x offset) rounded to real(4) (W/pm.f90:63):
                    both terms are exact floats, so one float add rounds it */
                 const float off4 = (float)(4.0 * cube_xoff(IX, code));
@@ -248,6 +284,7 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
                     s = s + wx * wy * wz * mass;
                   }
                 }
+              }
               }
             }
           }
@@ -311,17 +348,43 @@ __global__ __launch_bounds__(MT) void k_coarse_deposit_b2(const typename IntOf<I
     }
     float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int qz = 0; qz < 4; ++qz)
-      for (int sy = m0[1] - 1; sy <= m0[1] + 2; ++sy)
+      for (int sy = m0[1] - 1; sy <= m0[1] + 2; ++sy) {
+        /* the row's 4 source cells: counts and starts loaded together */
+        int npr[4];
+        int64_t p0r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t cc = cid_global(g, T[0] * g.nt + m0[0] - 2 + u, T[1] * g.nt + sy - 1, T[2] * g.nt + zs[qz] - 1);
+          npr[u] = rhoc[cc];
+          p0r[u] = cstart[cc];
+        }
+#pragma unroll 1
         for (int sx = m0[0] - 1; sx <= m0[0] + 2; ++sx) {
           const int ci[3] = {sx, sy, zs[qz]};
-          const int64_t cc = cid_global(g, T[0] * g.nt + ci[0] - 1, T[1] * g.nt + ci[1] - 1, T[2] * g.nt + ci[2] - 1);
-          const int np = rhoc[cc];
-          const int64_t p0 = cstart[cc];
-          for (int l = 0; l < np; ++l) {
+          const int ui = sx - (m0[0] - 1);
+          int np = npr[0];
+          int64_t p0 = p0r[0];
+#pragma unroll
+          for (int u = 1; u < 4; ++u) { /* register select, no dynamic indexing */
+            np = ui == u ? npr[u] : np;
+            p0 = ui == u ? p0r[u] : p0;
+          }
+          /* four particles' codes in flight at once (the last one repeated) */
+          for (int l0 = 0; l0 < np; l0 += 4) {
+          int32_t code4[4][3];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int64_t q = p0 + min(l0 + u, np - 1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) code4[u][a] = (int32_t)xp[3 * q + a];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (l0 + u >= np) break;
             float w0[3], w1[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-              const int32_t code = (int32_t)xp[3 * (p0 + l) + a];
+              const int32_t code = code4[u][a];
               const float tempx = (float)(((double)(ci[a] - 1) + cube_xoff(IX, code)) - 0.5);
               const int i1 = (int)floorf(tempx) + 1;
               const float d1 = (float)i1 - tempx, d2 = 1.0f - d1;
@@ -335,7 +398,9 @@ __global__ __launch_bounds__(MT) void k_coarse_deposit_b2(const typename IntOf<I
               s[q] = s[q] + wx * wy * wz * mass;
             }
           }
+          }
         }
+      }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int x = gg[0] + (q & 1), y = gg[1] + ((q >> 1) & 1), z = gg[2] + ((q >> 2) & 1);
@@ -394,6 +459,28 @@ int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc) {
    contiguous; each thread finds its cell by binary search in LDS */
 #define KT 256
 
+/* a/b for a divisor fixed at compile time, y = RN(1/b): q0 = RN(a*y) and
+   Markstein's correction RN(q0 + (a - b*q0)*y) with both steps fused.  For
+   b = 6 and b = pi in real(4) this equals the IEEE quotient bit for bit for
+   every 1e-30 <= |a| < 1e30 (checked on the CPU over every float of that
+   range by tests/test_divide_const.py); `ok` is cleared for other a, whose
+   caller redoes the sum with the division. */
+template <int B> struct FixedDiv;
+template <> struct FixedDiv<6> {
+  static constexpr float b = 6.0f, y = 1.0f / 6.0f;
+};
+template <> struct FixedDiv<314> {
+  static constexpr float b = CUBE_PI_F, y = 1.0f / CUBE_PI_F;
+};
+template <int B> __device__ __forceinline__ float div_fixed(float a, bool &ok) {
+  constexpr float b = FixedDiv<B>::b, y = FixedDiv<B>::y;
+  const float m = fabsf(a);
+  ok = ok && m >= 1e-30f && m < 1e30f;
+  const float q0 = a * y;
+  const float r = __builtin_fmaf(-q0, b, a);
+  return __builtin_fmaf(r, y, q0);
+}
+
 template <int IX, int IV, bool FINE>
 __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__restrict__ xp,
                                              typename IntOf<IV>::T *__restrict__ vp,
@@ -450,6 +537,9 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
     const int n = FINE ? nfine : g.nc;
     /* corner order of W/pm.f90:146-153 */
     const int cx[8] = {0, 1, 0, 0, 0, 1, 1, 1}, cy[8] = {0, 0, 1, 0, 1, 0, 1, 1}, cz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
+    /* vreal += ((((((F*a_mid)*dt)/6)/pi)*wx)*wy)*wz per corner, W/pm.f90:146-153 */
+    const double v0[3] = {vreal[0], vreal[1], vreal[2]};
+    bool ok = true;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int X = cx[q] ? m2[0] : m1[0], Y = cy[q] ? m2[1] : m1[1], Z = cz[q] ? m2[2] : m1[2];
@@ -457,8 +547,21 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
       const float *F = force + 3 * (((int64_t)Z * n + Y) * n + X);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float inc = ((((((F[a] * a_mid) * dt) / 6.0f) / CUBE_PI_F) * wx) * wy) * wz;
+        const float inc = ((div_fixed<314>(div_fixed<6>((F[a] * a_mid) * dt, ok), ok) * wx) * wy) * wz;
         vreal[a] = vreal[a] + (double)inc;
+      }
+    }
+    if (!ok) { /* a quotient outside the checked range: redo with divisions */
+#pragma unroll
+      for (int a = 0; a < 3; ++a) vreal[a] = v0[a];
+      for (int q = 0; q < 8; ++q) {
+        const int X = cx[q] ? m2[0] : m1[0], Y = cy[q] ? m2[1] : m1[1], Z = cz[q] ? m2[2] : m1[2];
+        const float wx = cx[q] ? d2[0] : d1[0], wy = cy[q] ? d2[1] : d1[1], wz = cz[q] ? d2[2] : d1[2];
+        const float *F = force + 3 * (((int64_t)Z * n + Y) * n + X);
+        for (int a = 0; a < 3; ++a) {
+          const float inc = ((((((F[a] * a_mid) * dt) / 6.0f) / CUBE_PI_F) * wx) * wy) * wz;
+          vreal[a] = vreal[a] + (double)inc;
+        }
       }
     }
 #pragma unroll

@@ -90,6 +90,14 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIBPATH):
             raise RuntimeError(f"{LIBPATH} missing: build it with `python -m cafproject_amd.build`")
+        # torch ships its own HIP and HSA runtimes under the same soname.
+        # Loaded first, they also serve this library (one runtime per
+        # process); loaded after it, they start a second runtime that finds
+        # no device.  So torch, when installed, is imported before the load.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIBPATH)
         P, i64, f32, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float, ctypes.c_int
         L.cube_ctx_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(P)]
