@@ -28,7 +28,7 @@ extern "C" const char *cube_strerror(int code) {
 static void free_all(cube_ctx *c) {
   void *ptrs[] = {c->xp, c->vp, c->xp2, c->vp2, c->pid, c->pid2, c->rhoc, c->rhoc2, c->vfield, c->vfield2,
                   c->cstart, c->cstart2, c->dest, c->key, c->sortbuf, c->fill, c->xdest, c->xkey, c->rank, c->xrank, c->fbflag,
-                  c->tanf_tab, c->vdec0, c->vdec1, c->rho_f, c->spec_f, c->specd_f, c->force_f,
+                  c->tanf_tab, c->venc_tab, c->vdec0, c->vdec1, c->rho_f, c->spec_f, c->specd_f, c->force_f,
                   c->kern_f, c->rho_c, c->spec_c, c->specd_c, c->force_c, c->kern_c, c->dred,
                   c->ured, c->ired, c->scanbuf, c->vpart};
   for (void *p : ptrs)
@@ -121,6 +121,7 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->xrank, c->xcap);
   st |= dalloc(&c->fbflag, c->ncells / 64 + 2); /* one per 64-cell finalize workgroup of the drift */
   st |= dalloc(&c->tanf_tab, c->nvbin);
+  st |= dalloc(&c->venc_tab, c->nvbin / 2);
   st |= dalloc(&c->vdec0, c->nvbin);
   st |= dalloc(&c->vdec1, c->nvbin);
   const int64_t nf3 = c->nf * c->nf * c->nf, nspec = (int64_t)cube_spec_pitch((int)c->nf) * c->nf * c->nf;
@@ -162,8 +163,17 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
       const int code = i - nv / 2;
       h[i] = tanf((CUBE_PI_F * (float)code) / nvm1);
     }
-    const hipError_t e = hipMemcpy(c->tanf_tab, h, nv * sizeof(float), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(c->tanf_tab, h, nv * sizeof(float), hipMemcpyHostToDevice);
     free(h);
+    /* code boundaries of the encoder: tan((k+0.5)*pi/(nvbin-1)), k < nv/2-1,
+       in long double (the last one, beyond pi/2, is +inf) */
+    const int kmax = nv / 2 - 1;
+    double *tb = (double *)malloc((size_t)(kmax + 1) * sizeof(double));
+    for (int k = 0; k < kmax; ++k)
+      tb[k] = (double)tanl(((long double)k + 0.5L) * (long double)CUBE_PI_F / (long double)(nv - 1));
+    tb[kmax] = HUGE_VAL;
+    if (e == hipSuccess) e = hipMemcpy(c->venc_tab, tb, (size_t)(kmax + 1) * sizeof(double), hipMemcpyHostToDevice);
+    free(tb);
     if (e != hipSuccess) {
       free_all(c);
       delete c;

@@ -66,6 +66,7 @@ struct cube_ctx {
   int64_t xcap;
   /* codec tables */
   float *tanf_tab;          /* nvbin: tanf(pi*code/(nvbin-1)), index code+nvbin/2 */
+  double *venc_tab;         /* nvbin/2: velocity code boundaries, cube_encode_v_tab */
   double *vdec0, *vdec1;    /* nvbin: decoded residual velocity for two sigmas  */
   /* fine mesh */
   float *rho_f;             /* nf^3 */
@@ -136,6 +137,29 @@ __device__ static inline int64_t cid_global(const Geo &g, int gx, int gy, int gz
 template <int B> struct IntOf;
 template <> struct IntOf<1> { typedef int8_t T; };
 template <> struct IntOf<2> { typedef int16_t T; };
+
+/* cube_encode_v through the code boundaries T[k] = tan((k+0.5)*pi/(nvbin-1))
+   for k < kmax and T[kmax] = +inf (venc_tab, built on the host in long
+   double; pi is the real(4) pi of the reference): the code of |u| is the
+   number of boundaries <= |u| (nint rounds half away from zero), and atan
+   being odd makes the negative side its mirror.  An FP32 atan, within 0.01
+   of the code, picks the candidate and two table reads settle it.  This is
+   nint((nvbin-1)*atan(u)/pi) in exact arithmetic; the FP64 formula rounds
+   to the same integer except within about 1e-11 of a half-integer. */
+template <int IV>
+__device__ __forceinline__ int32_t cube_encode_v_tab(double c, double v, const double *__restrict__ T) {
+  constexpr int kmax = IV == 1 ? 127 : 32767;
+  constexpr float scale = (float)((IV == 1 ? 255.0 : 65535.0) / (double)CUBE_PI_F);
+  const double u = c * v;
+  const double a = fabs(u);
+  int k0 = (int)(scale * atanf((float)a) + 0.5f);
+  k0 = k0 < kmax ? k0 : kmax;
+  const double lo = k0 > 0 ? T[k0 - 1] : -1.0;
+  const double hi = T[k0];
+  int k = k0 - (a < lo ? 1 : 0) + (a >= hi ? 1 : 0);
+  k = k < 0 ? 0 : (k > kmax ? kmax : k);
+  return cube_wrap(IV, u < 0 ? -k : k);
+}
 
 /* Half spectra of an n^3 real mesh are stored as rows of n/2+1 complex
    values padded to a multiple of 16 (128 bytes), so every row and every

@@ -134,15 +134,24 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
  * neighbourhood (16+2*pr fine cells per axis) are staged in LDS, bucketed
  * per fine cell in chain order (decreasing l, the order of the reference's
  * hoc/ll lists), as exact fixed-point positions X = x * 2^15 in the tile
- * frame (x = ncell*(cell-1) + ncell*offset needs at most 10+15 bits).
- * x2 - x1 = (X2 - X1) * 2^-15 is then the same FP64 value the reference
- * forms, and each particle sums its pairs in the same order, so the result
- * is bit-identical to the unstaged walk.  A block whose neighbourhood holds
- * more than `maxs` particles walks the global lists instead.
+ * frame (x = ncell*(cell-1) + ncell*offset needs at most 10+15 bits), kept
+ * relative to the region's corner (under 2^20 per axis) and packed 21 bits
+ * per axis into one 64-bit word.  x2 - x1 = (X2 - X1) * 2^-15 is then the
+ * same FP64 value the reference forms, and each particle sums its pairs in
+ * the same order, so the result is bit-identical to the unstaged walk.
+ *
+ * A fine cell lies in exactly one coarse cell, so one thread per staged
+ * coarse cell counts its particles into the per-fine-cell table and, after
+ * the scan, places them walking l downwards, with plain LDS updates.  A
+ * block whose neighbourhood holds more than `maxs` particles walks the
+ * global lists instead.
  */
 #define PPB_T 256
 __device__ static inline int pp_fix(int IX, int cc, int32_t code) {
   return (cc - 1) * (4 * 32768) + (int)(2 * cube_ucode(IX, code) + 1) * (IX == 1 ? 256 : 1);
+}
+__device__ static inline uint64_t pp_pack(const int r[3]) {
+  return (uint64_t)(uint32_t)r[0] | ((uint64_t)(uint32_t)r[1] << 21) | ((uint64_t)(uint32_t)r[2] << 42);
 }
 
 template <int IX, int IV>
@@ -150,19 +159,16 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
                                                   typename IntOf<IV>::T *__restrict__ vp,
                                                   const int32_t *__restrict__ rhoc,
                                                   const int64_t *__restrict__ cstart,
-                                                  const double *__restrict__ vdec, Geo g, int nvh, int pr,
+                                                  const double *__restrict__ vdec, const double *__restrict__ vtab, Geo g, int nvh, int pr,
                                                   double rsoft, double mass, double a_mid, double dt, double venc,
                                                   int maxs, unsigned long long *__restrict__ out) {
-  extern __shared__ int smem[];
+  extern __shared__ uint64_t smem64[];
   const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
-  int *cur = smem;                  /* per fine cell: count, then start offset (nreg+1) */
-  int *pos = smem + nreg + 1;       /* staged positions, 3 ints each (maxs) */
-  int16_t *sfi = reinterpret_cast<int16_t *>(pos + 3 * maxs); /* per walked particle: region cell or -1 */
+  uint64_t *pos = smem64;                                            /* packed positions (maxs) */
+  uint16_t *cur = reinterpret_cast<uint16_t *>(smem64 + maxs) + 1; /* per fine cell, cur[-1] = 0 */
   __shared__ int part[PPB_T];
   __shared__ int64_t cst[16][5];
   __shared__ int rowoff[17];
-  __shared__ int coff[217];         /* particles of the 216 staged cells: exclusive offsets */
-  __shared__ int64_t cq0[216];      /* their first particle */
   const int tid = threadIdx.x, nt = g.nt, nb = nt / 4, nnt = g.nnt;
   const int sh = IX == 1 ? 6 : 14;
   int64_t b = blockIdx.x;
@@ -171,66 +177,46 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   const int tt[3] = {t % nnt, (t / nnt) % nnt, t / (nnt * nnt)};
   const int B0[3] = {4 * bx, 4 * by, 4 * bz};             /* cells B0+1..B0+4 (1-based, tile frame) */
   const int flo[3] = {4 * B0[0] + 1 - pr, 4 * B0[1] + 1 - pr, 4 * B0[2] + 1 - pr};
+  /* tile-frame fixed-point coordinate of the region's low corner */
+  const int X0[3] = {(flo[0] - 1) * 32768, (flo[1] - 1) * 32768, (flo[2] - 1) * 32768};
 
-  for (int i = tid; i < nreg; i += PPB_T) cur[i] = 0;
+  for (int i = tid - 1; i < nreg; i += PPB_T) cur[i] = 0;
   if (tid < 16) {
     const int ry = tid % 4, rz = tid / 4;
     const int64_t c0 = g.cid(B0[0], B0[1] + ry, B0[2] + rz, t);
 #pragma unroll
     for (int q = 0; q < 5; ++q) cst[tid][q] = cstart[c0 + q];
   }
-  /* the staged coarse cells B0..B0+5 per axis (tile frame, 1-based):
-     counts, first particles, and a scan of the counts */
-  int snp = 0;
-  if (tid < 216) {
-    const int sc[3] = {B0[0] + tid % 6, B0[1] + (tid / 6) % 6, B0[2] + tid / 36};
+  /* the staged coarse cells B0..B0+5 per axis (tile frame, 1-based) */
+  const int q = tid;
+  int sc[3] = {0, 0, 0}, np = 0;
+  int64_t p0 = 0;
+  if (q < 216) {
+    sc[0] = B0[0] + q % 6;
+    sc[1] = B0[1] + (q / 6) % 6;
+    sc[2] = B0[2] + q / 36;
     const int64_t cq = cid_global(g, tt[0] * nt + sc[0] - 1, tt[1] * nt + sc[1] - 1, tt[2] * nt + sc[2] - 1);
-    snp = rhoc[cq];
-    cq0[tid] = cstart[cq];
+    np = rhoc[cq];
+    p0 = cstart[cq];
   }
-  part[tid] = snp;
   __syncthreads();
-  for (int o = 1; o < PPB_T; o <<= 1) {
-    const int v = tid >= o ? part[tid - o] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  if (tid < 216) coff[tid + 1] = part[tid];
-  if (tid == 0) coff[0] = 0;
-  __syncthreads();
-  const int nwalk = coff[216];
-  const bool fits = nwalk <= 2 * maxs; /* sfi holds 2*maxs entries; uniform over the block */
-  /* the staged cell and sub-cell region index of walked particle j */
-  auto locate = [&](int j, int &q, int &l, int32_t code[3], int &ri) {
-    int lo = 0, hi = 216;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (coff[mid] <= j) lo = mid; else hi = mid;
-    }
-    q = lo;
-    l = j - coff[q];
-    const int sc[3] = {B0[0] + q % 6, B0[1] + (q / 6) % 6, B0[2] + q / 36};
-    int r[3];
+  /* the region cell of particle l of this thread's staged cell, or -1 */
+  auto region = [&](int l, int32_t code[3]) -> int {
     bool in = true;
+    int r[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      code[a] = (int32_t)xp[3 * (cq0[q] + l) + a];
-      const int f = 4 * (sc[a] - 1) + (int)(cube_ucode(IX, code[a]) >> sh) + 1;
-      r[a] = f - flo[a];
+      code[a] = (int32_t)xp[3 * (p0 + l) + a];
+      r[a] = 4 * (sc[a] - 1) + (int)(cube_ucode(IX, code[a]) >> sh) + 1 - flo[a];
       in = in && r[a] >= 0 && r[a] < RF;
     }
-    ri = in ? (r[2] * RF + r[1]) * RF + r[0] : -1;
+    return in ? (r[2] * RF + r[1]) * RF + r[0] : -1;
   };
-  /* count per fine cell, all walked particles in parallel */
-  if (fits) {
-    for (int j = tid; j < nwalk; j += PPB_T) {
-      int q, l, ri;
-      int32_t code[3];
-      locate(j, q, l, code, ri);
-      sfi[j] = (int16_t)ri;
-      if (ri >= 0) atomicAdd(cur + ri, 1);
-    }
+  /* count per fine cell: the cell's owner thread only, no atomics */
+  for (int l = 0; l < np; ++l) {
+    int32_t code[3];
+    const int ri = region(l, code);
+    if (ri >= 0) cur[ri] = (uint16_t)(cur[ri] + 1);
   }
   __syncthreads();
   /* exclusive scan of the counts */
@@ -247,42 +233,39 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     __syncthreads();
   }
   const int nstage = part[PPB_T - 1];
+  const bool staged = nstage <= maxs; /* uniform over the block */
   s = part[tid] - s;
   for (int i = i0; i < i1; ++i) {
     const int cnt = cur[i];
-    cur[i] = s;
+    cur[i] = (uint16_t)(staged ? s : 0);
     s += cnt;
   }
-  if (tid == 0) cur[nreg] = nstage;
   __syncthreads();
-  const bool staged = fits && nstage <= maxs; /* uniform over the block */
   if (staged) {
-    /* chain order inside a fine cell is decreasing l: a particle's slot is
-       its cell's start plus the number of same-cell particles with larger l
-       (all in the same coarse cell) */
-    for (int j = tid; j < nwalk; j += PPB_T) {
-      if (sfi[j] < 0) continue;
-      int q, l, ri;
+    /* chain order inside a fine cell is decreasing l: walk l downwards and
+       append; afterwards cur[f] is the end of fine cell f's list */
+    for (int l = np - 1; l >= 0; --l) {
       int32_t code[3];
-      locate(j, q, l, code, ri);
-      int rank = 0;
-      for (int j2 = j + 1; j2 < coff[q + 1]; ++j2) rank += sfi[j2] == ri;
-      const int slot = cur[ri] + rank;
-      const int sc[3] = {B0[0] + q % 6, B0[1] + (q / 6) % 6, B0[2] + q / 36};
+      const int ri = region(l, code);
+      if (ri < 0) continue;
+      const int slot = cur[ri];
+      cur[ri] = (uint16_t)(slot + 1);
+      int r[3];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) pos[3 * slot + a] = pp_fix(IX, sc[a], code[a]);
+      for (int a = 0; a < 3; ++a) r[a] = pp_fix(IX, sc[a], code[a]) - X0[a];
+      pos[slot] = pp_pack(r);
     }
   }
   if (tid == 0) {
     int acc = 0;
-    for (int q = 0; q < 16; ++q) {
-      rowoff[q] = acc;
-      acc += (int)(cst[q][4] - cst[q][0]);
+    for (int q2 = 0; q2 < 16; ++q2) {
+      rowoff[q2] = acc;
+      acc += (int)(cst[q2][4] - cst[q2][0]);
     }
     rowoff[16] = acc;
   }
   __syncthreads();
-  /* fine cell f's list is pos[cur[f] .. cur[f+1]) */
+  /* fine cell f's list is pos[cur[f-1] .. cur[f]) */
 
   double f2m = 0;
   unsigned long long ntest = 0;
@@ -300,33 +283,55 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     for (int a = 0; a < 3; ++a) {
       const int32_t code = (int32_t)xp[3 * p + a];
       x1[a] = (double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, code);
-      X1[a] = pp_fix(IX, loc[a], code);
+      X1[a] = pp_fix(IX, loc[a], code) - X0[a];
       gf[a] = 4 * (loc[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
       vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
     }
+    if (staged) {
+      /* the 2pr+1 fine cells ii-pr..ii+pr of one (kk,jj) are adjacent in
+         the region order, so their lists form one contiguous range of pos:
+         (2pr+1)^2 ranges, walked kk-major then jj, hold every neighbour in
+         the reference's (kk,jj,ii, chain) order.  One flat loop over the
+         particle's pairs keeps a wave's lanes together: it runs the largest
+         pair count of the wave, not 27 times the largest per-cell count. */
+      const int RW = 2 * pr + 1;
+      const int f00 = ((gf[2] - pr - flo[2]) * RF + (gf[1] - pr - flo[1])) * RF + (gf[0] - pr - flo[0]);
+      int n = 0;
+      for (int r = 0; r < RW * RW; ++r) {
+        const int f0 = f00 + ((r / RW) * RF + r % RW) * RF;
+        n += (int)cur[f0 + RW - 1] - (int)cur[f0 - 1];
+      }
+      int r = -1, base = 0, cend = 0; /* pair t < cend lives at pos slot base + t */
+      for (int t = 0; t < n; ++t) {
+        while (t >= cend) {
+          ++r;
+          const int f0 = f00 + ((r / RW) * RF + r % RW) * RF;
+          const int s0 = cur[f0 - 1], e = cur[f0 + RW - 1];
+          base = s0 - cend;
+          cend += e - s0;
+        }
+        const uint64_t w = pos[base + t];
+        const int X2[3] = {(int)(w & 0x1FFFFF), (int)((w >> 21) & 0x1FFFFF), (int)(w >> 42)};
+        double x21[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) x21[a] = (double)(X2[a] - X1[a]) * (1.0 / 32768.0);
+        double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
+        if (rmag == 0.0) rmag = 1.0;
+        const double rcut = rmag / 16.0;
+        const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
+        const double r3 = powi3(rmag);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          double fp = (mass * (x21[a] / r3)) * pcut;
+          if (!(rmag > rsoft)) fp = fp * 0.0;
+          ft[a] = ft[a] + fp;
+        }
+      }
+    } else {
     for (int kk = gf[2] - pr; kk <= gf[2] + pr; ++kk)
       for (int jj = gf[1] - pr; jj <= gf[1] + pr; ++jj)
         for (int ii = gf[0] - pr; ii <= gf[0] + pr; ++ii) {
-          if (staged) {
-            const int f = ((kk - flo[2]) * RF + (jj - flo[1])) * RF + (ii - flo[0]);
-            const int s0 = cur[f], e = cur[f + 1];
-            for (int q = s0; q < e; ++q) {
-              double x21[3];
-#pragma unroll
-              for (int a = 0; a < 3; ++a) x21[a] = (double)(pos[3 * q + a] - X1[a]) * (1.0 / 32768.0);
-              double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
-              if (rmag == 0.0) rmag = 1.0;
-              const double rcut = rmag / 16.0;
-              const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
-              const double r3 = powi3(rmag);
-#pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                double fp = (mass * (x21[a] / r3)) * pcut;
-                if (!(rmag > rsoft)) fp = fp * 0.0;
-                ft[a] = ft[a] + fp;
-              }
-            }
-          } else {
+          {
             const int fc[3] = {ii, jj, kk};
             int cc[3], sub[3];
 #pragma unroll
@@ -362,10 +367,11 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
             }
           }
         }
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       vreal[a] = vreal[a] + (((ft[a] * a_mid) * dt) / 6.0) / (double)CUBE_PI_F;
-      vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal[a]);
+      vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v_tab<IV>(venc, vreal[a], vtab);
     }
     const double f2 = (ft[0] * ft[0] + ft[1] * ft[1]) + ft[2] * ft[2];
     f2m = f2 > f2m ? f2 : f2m;
@@ -373,8 +379,8 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   }
   /* block reduction through the (now free) staging area */
   __syncthreads();
-  double *red = reinterpret_cast<double *>(smem);
-  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(smem) + PPB_T;
+  double *red = reinterpret_cast<double *>(smem64);
+  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(smem64) + PPB_T;
   red[tid] = f2m;
   cnt[tid] = ntest;
   __syncthreads();
@@ -402,15 +408,17 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
     /* 64 KB of LDS per workgroup: the fine-cell table, the staged positions
        (3 ints each) and the walked particles' cell indices (2 per int) */
     const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
-    const int dyn_ints = (65536 - 4608) / 4;
-    int maxs = (dyn_ints - (nreg + 1)) / 4;
+    /* staged positions (8 bytes each) and the uint16 fine-cell table in
+       about 28 KB, so five workgroups share a CU's LDS */
+    int maxs = pr <= 1 ? 2048 : (pr == 2 ? 1024 : 512);
     /* a lower cap sends more workgroups down the unstaged walk (used by the
        tests to cover both walks in one run) */
     if (const char *e = getenv("CUBE_PP_STAGE_MAX")) maxs = std::min(maxs, atoi(e));
     const int64_t nb = c->nt / 4, nblk = nb * nb * nb * c->ntiles;
-    hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3((unsigned)nblk), dim3(PPB_T), (size_t)dyn_ints * 4, c->stream,
+    const size_t dyn = (size_t)maxs * 8 + (((size_t)nreg + 1) * 2 + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3((unsigned)nblk), dim3(PPB_T), dyn, c->stream,
                        (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
-                       c->vdec0, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p, (double)a_mid,
+                       c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p, (double)a_mid,
                        (double)dt, cube_vconst(sigma_vi), maxs, out);
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
