@@ -80,32 +80,35 @@ CUBE_HD static inline double cube_atan(double x) {
     return neg ? -r : r;
   }
   if (ax < 3.725290298461914e-09) return x; /* 2^-28 */
-  int id;
-  double t;
-  if (ax < 0.4375) {
-    id = -1;
-    t = ax;
-  } else if (ax < 0.6875) {
-    id = 0;
-    t = (2.0 * ax - 1.0) / (2.0 + ax);
-  } else if (ax < 1.1875) {
-    id = 1;
-    t = (ax - 1.0) / (ax + 1.0);
-  } else if (ax < 2.4375) {
-    id = 2;
-    t = (ax - 1.5) / (1.0 + 1.5 * ax);
-  } else {
+  /* the reduced argument t = num/den of the interval, chosen by selects so
+     that one division serves every lane */
+  double num = ax, den = 1.0, h = 0.0, l = 0.0;
+  int id = -1;
+  if (ax >= 2.4375) {
     id = 3;
-    t = -1.0 / ax;
+    num = -1.0;
+    den = ax;
+  } else if (ax >= 1.1875) {
+    id = 2;
+    num = ax - 1.5;
+    den = 1.0 + 1.5 * ax;
+  } else if (ax >= 0.6875) {
+    id = 1;
+    num = ax - 1.0;
+    den = ax + 1.0;
+  } else if (ax >= 0.4375) {
+    id = 0;
+    num = 2.0 * ax - 1.0;
+    den = 2.0 + ax;
   }
+  if (id >= 0) {
+    h = hi[id];
+    l = lo[id];
+  }
+  const double t = num / den; /* ax/1 = ax exactly on the first interval */
   const double z = t * t;
   const double p = a0 + z * (a1 + z * (a2 + z * (a3 + z * (a4 + z * (a5 + z * (a6 + z * (a7 + z * (a8 + z * (a9 + z * a10)))))))));
-  double r;
-  if (id < 0) {
-    r = t - t * (z * p);
-  } else {
-    r = hi[id] - ((t * (z * p) - lo[id]) - t);
-  }
+  const double r = id < 0 ? t - t * (z * p) : h - ((t * (z * p) - l) - t);
   return neg ? -r : r;
 }
 

@@ -239,52 +239,86 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
               np = qi == q ? npc[q] : np;
               p0 = qi == q ? p0c[q] : p0;
             }
-            /* four particles' codes are loaded together (the last one
-               repeated past the end) so their loads are in flight at once */
-            for (int l0 = 0; l0 < np; l0 += 4) {
-              int32_t code4[4][3];
+            /* a source cell below C on an axis reaches C's 4^3 fine cells
+               only through particles in its last fine sub-cell on that axis
+               (code >> sh == 3; the rounding of tempx near the cell's end
+               keeps that code), so those visits first mark the candidates
+               of each 64-particle chunk in a mask and then visit only them,
+               in increasing l */
+            const int sh = IX == 1 ? 6 : 14;
+            const bool ex = qx == 0, ey = qy == 0, ez = ci[2] == C[2] - 1;
+            for (int lb = 0; lb < np; lb += 64) {
+              const int nb = min(64, np - lb);
+              uint64_t mask = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
+              if (ex || ey || ez) {
+                uint64_t m = 0;
+                for (int l0 = 0; l0 < nb; l0 += 4) {
+                  int32_t c4[4][3];
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const int64_t q = p0 + min(l0 + u, np - 1);
+                  for (int u = 0; u < 4; ++u) {
+                    const int64_t q = p0 + lb + min(l0 + u, nb - 1);
 #pragma unroll
-                for (int a = 0; a < 3; ++a) code4[u][a] = (int32_t)xp[3 * q + a];
-              }
+                    for (int a = 0; a < 3; ++a) c4[u][a] = (int32_t)xp[3 * q + a];
+                  }
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
-              if (l0 + u >= np) break;
-              int r[3];
-              float d1[3];
-#pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                const int32_t code = code4[u][a];
-                /* ncell*(i-1)+ncell*(x offset) rounded to real(4) (W/pm.f90:63):
-                   both terms are exact floats, so one float add rounds it */
-                const float off4 = (float)(4.0 * cube_xoff(IX, code));
-                const float tempx = (float)(4 * (ci[a] - 1)) + off4;
-                const int i1 = (int)floorf(tempx) + 1;
-                d1[a] = (float)i1 - tempx;
-                r[a] = i1 - 4 * (C[a] - 1) - 1; /* 0-based in C's block */
-              }
-#pragma unroll
-              for (int cz = 0; cz < 2; ++cz) {
-                const int z = r[2] + cz;
-                if (z < 0 || z > 3) continue;
-                const float wz = cz ? 1.0f - d1[2] : d1[2];
-#pragma unroll
-                for (int cy = 0; cy < 2; ++cy) {
-                  const int y = r[1] + cy;
-                  if (y < 0 || y > 3) continue;
-                  const float wy = cy ? 1.0f - d1[1] : d1[1];
-#pragma unroll
-                  for (int cx = 0; cx < 2; ++cx) {
-                    const int x = r[0] + cx;
-                    if (x < 0 || x > 3) continue;
-                    const float wx = cx ? 1.0f - d1[0] : d1[0];
-                    float &s = acc[((z * 4 + y) * 4 + x) * FD_T + t];
-                    s = s + wx * wy * wz * mass;
+                  for (int u = 0; u < 4; ++u) {
+                    const bool pass = (!ex || (cube_ucode(IX, c4[u][0]) >> sh) == 3) &&
+                                      (!ey || (cube_ucode(IX, c4[u][1]) >> sh) == 3) &&
+                                      (!ez || (cube_ucode(IX, c4[u][2]) >> sh) == 3);
+                    m |= (uint64_t)(pass ? 1 : 0) << (l0 + u);
                   }
                 }
+                mask &= m;
               }
+              /* two candidates per round so that both code loads are in flight */
+              while (mask) {
+                int lq[2];
+                lq[0] = lb + __builtin_ctzll(mask);
+                mask &= mask - 1ull;
+                const bool two = mask != 0ull;
+                lq[1] = two ? lb + __builtin_ctzll(mask) : lq[0];
+                if (two) mask &= mask - 1ull;
+                int32_t cq[2][3];
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                  for (int a = 0; a < 3; ++a) cq[u][a] = (int32_t)xp[3 * (p0 + lq[u]) + a];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                  if (u == 1 && !two) break;
+                  int r[3];
+                  float d1[3];
+#pragma unroll
+                  for (int a = 0; a < 3; ++a) {
+                    /* ncell*(i-1)+ncell*(x offset) rounded to real(4) (W/pm.f90:63):
+                       both terms are exact floats, so one float add rounds it */
+                    const float off4 = (float)(4.0 * cube_xoff(IX, cq[u][a]));
+                    const float tempx = (float)(4 * (ci[a] - 1)) + off4;
+                    const int i1 = (int)floorf(tempx) + 1;
+                    d1[a] = (float)i1 - tempx;
+                    r[a] = i1 - 4 * (C[a] - 1) - 1; /* 0-based in C's block */
+                  }
+#pragma unroll
+                  for (int cz = 0; cz < 2; ++cz) {
+                    const int z = r[2] + cz;
+                    if (z < 0 || z > 3) continue;
+                    const float wz = cz ? 1.0f - d1[2] : d1[2];
+#pragma unroll
+                    for (int cy = 0; cy < 2; ++cy) {
+                      const int y = r[1] + cy;
+                      if (y < 0 || y > 3) continue;
+                      const float wy = cy ? 1.0f - d1[1] : d1[1];
+#pragma unroll
+                      for (int cx = 0; cx < 2; ++cx) {
+                        const int x = r[0] + cx;
+                        if (x < 0 || x > 3) continue;
+                        const float wx = cx ? 1.0f - d1[0] : d1[0];
+                        float &sacc = acc[((z * 4 + y) * 4 + x) * FD_T + t];
+                        sacc = sacc + wx * wy * wz * mass;
+                      }
+                    }
+                  }
+                }
               }
             }
           }
@@ -463,8 +497,7 @@ int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc) {
    Markstein's correction RN(q0 + (a - b*q0)*y) with both steps fused.  For
    b = 6 and b = pi in real(4) this equals the IEEE quotient bit for bit for
    every 1e-30 <= |a| < 1e30 (checked on the CPU over every float of that
-   range by tests/test_divide_const.py); `ok` is cleared for other a, whose
-   caller redoes the sum with the division. */
+   range by tests/test_divide_const.py). */
 template <int B> struct FixedDiv;
 template <> struct FixedDiv<6> {
   static constexpr float b = 6.0f, y = 1.0f / 6.0f;
@@ -472,13 +505,31 @@ template <> struct FixedDiv<6> {
 template <> struct FixedDiv<314> {
   static constexpr float b = CUBE_PI_F, y = 1.0f / CUBE_PI_F;
 };
-template <int B> __device__ __forceinline__ float div_fixed(float a, bool &ok) {
+template <int B> __device__ __forceinline__ float div_fixed(float a) {
   constexpr float b = FixedDiv<B>::b, y = FixedDiv<B>::y;
-  const float m = fabsf(a);
-  ok = ok && m >= 1e-30f && m < 1e30f;
   const float q0 = a * y;
   const float r = __builtin_fmaf(-q0, b, a);
   return __builtin_fmaf(r, y, q0);
+}
+/* ((a/6)/pi) for the kicks: when 6e-30 <= |a| < 1e30 both quotients stay in
+   the checked range (a/6 lands in [1e-30, 1.7e29)); `ok` is cleared
+   otherwise and the caller redoes the sum with divisions. */
+__device__ __forceinline__ float div_6pi(float a, bool &ok) {
+  const float m = fabsf(a);
+  ok = ok && m >= 6e-30f && m < 1e30f;
+  return div_fixed<314>(div_fixed<6>(a));
+}
+
+/* G = (((F*a_mid)*dt)/6)/pi per mesh value, the factor of every kick term
+   (W/pm.f90:341-348); the coarse kick reads G so each coarse cell's value is
+   formed once instead of by every particle whose cloud touches it */
+__global__ void k_kick_prescale(const float *__restrict__ F, int64_t n, float a_mid, float dt, float *__restrict__ G) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const float a = (F[q] * a_mid) * dt;
+    bool ok = true;
+    const float v = div_6pi(a, ok);
+    G[q] = ok ? v : (a / 6.0f) / CUBE_PI_F;
+  }
 }
 
 template <int IX, int IV, bool FINE>
@@ -537,7 +588,8 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
     const int n = FINE ? nfine : g.nc;
     /* corner order of W/pm.f90:146-153 */
     const int cx[8] = {0, 1, 0, 0, 0, 1, 1, 1}, cy[8] = {0, 0, 1, 0, 1, 0, 1, 1}, cz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
-    /* vreal += ((((((F*a_mid)*dt)/6)/pi)*wx)*wy)*wz per corner, W/pm.f90:146-153 */
+    /* vreal += ((((((F*a_mid)*dt)/6)/pi)*wx)*wy)*wz per corner, W/pm.f90:146-153;
+       the coarse mesh arrives as G = (((F*a_mid)*dt)/6)/pi */
     const double v0[3] = {vreal[0], vreal[1], vreal[2]};
     bool ok = true;
 #pragma unroll
@@ -547,11 +599,12 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
       const float *F = force + 3 * (((int64_t)Z * n + Y) * n + X);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float inc = ((div_fixed<314>(div_fixed<6>((F[a] * a_mid) * dt, ok), ok) * wx) * wy) * wz;
+        const float gq = FINE ? div_6pi((F[a] * a_mid) * dt, ok) : F[a];
+        const float inc = ((gq * wx) * wy) * wz;
         vreal[a] = vreal[a] + (double)inc;
       }
     }
-    if (!ok) { /* a quotient outside the checked range: redo with divisions */
+    if (FINE && !ok) { /* a quotient outside the checked range: redo with divisions */
 #pragma unroll
       for (int a = 0; a < 3; ++a) vreal[a] = v0[a];
       for (int q = 0; q < 8; ++q) {
@@ -626,7 +679,11 @@ int cube_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *
   unsigned long long *vmb = (unsigned long long *)c->ured + 8;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, vmb, 3);
   CUBE_LAUNCH_CHECK();
-  st = kick<false>(c, c->force_c, a_mid, dt, cube_vconst(sigma_vi), vmb);
+  /* G in the coarse spectra scratch, free once force_c is formed */
+  float *G = reinterpret_cast<float *>(c->specd_c);
+  hipLaunchKernelGGL(k_kick_prescale, dim3(2048), dim3(256), 0, c->stream, c->force_c, 3 * c->ncells, a_mid, dt, G);
+  CUBE_LAUNCH_CHECK();
+  st = kick<false>(c, G, a_mid, dt, cube_vconst(sigma_vi), vmb);
   if (st) return st;
   unsigned long long h[3];
   CUBE_CHECK(hipMemcpyAsync(h, vmb, sizeof(h), hipMemcpyDeviceToHost, c->stream));
