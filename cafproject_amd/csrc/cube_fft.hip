@@ -416,7 +416,10 @@ __global__ __launch_bounds__(RC<N>::NT) void k_c2c_r(float2 *__restrict__ a, con
 }
 
 /* Green's function pass along z, register version of k_green_z; OCT selects
-   the octant kernel storage (fine mesh) over the full one (coarse mesh) */
+   the octant kernel storage (fine mesh) over the full one (coarse mesh).
+   A workgroup does the lines y and N-y (which read the same kern_f rows of
+   the octant) for two adjacent column blocks (which share kern_f's 128-byte
+   lines), one after the other, so those reads come from its XCD's L2. */
 template <int N, bool OCT>
 __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict__ spec, float2 *__restrict__ outd,
                                                      const float *__restrict__ kern,
@@ -433,49 +436,58 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
   __shared__ float2 keep[KL > 0 ? KL * G::NT : 1];
   const int tid = threadIdx.x, c = tid % G::CB, t = tid / G::CB;
   for (int e = tid; e < N; e += G::NT) twl[e] = tw[e];
-  const int kx0 = blockIdx.x * G::CB + c, y = blockIdx.y;
-  const bool ok = kx0 < NH;
-  const int kx = ok ? kx0 : NH - 1;
-  const uint32_t lo = (uint32_t)(((int64_t)t * stride + kx) * 8);
-  const float2 *src = spec + (int64_t)y * SP;
-  float2 v[R2];
-  const int s0 = opaque_zero();
-#pragma unroll
-  for (int m = 0; m < R2; ++m) v[m] = *boff(uni(src + (int64_t)(R1 * m) * stride, s0), lo);
-  __syncthreads();
-  blk_fft<N, -1>(v, c, t, S, twl);
-#pragma unroll
-  for (int m = R2 - KL; m < R2; ++m) keep[(m - (R2 - KL)) * G::NT + tid] = v[m];
-  const int ky = y <= H ? y : N - y;
-  const float sy = y <= H ? 1.f : -1.f;
+  const int yp = blockIdx.y;
+  const int ny = (yp == 0 || yp == H) ? 1 : 2;
 #pragma unroll 1
-  for (int d = 0; d < 3; ++d) {
-    const int z0 = opaque_zero();
-    float2 w[R2];
+  for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll 1
+    for (int iy = 0; iy < ny; ++iy) {
+      const int y = iy == 0 ? yp : N - yp;
+      const int kx0 = (2 * blockIdx.x + cb) * G::CB + c;
+      const bool ok = kx0 < NH;
+      const int kx = ok ? kx0 : NH - 1;
+      const uint32_t lo = (uint32_t)(((int64_t)t * stride + kx) * 8);
+      const float2 *src = spec + (int64_t)y * SP;
+      float2 v[R2];
+      const int s0 = opaque_zero();
+      __syncthreads(); /* the previous line's keep/S reads are done */
 #pragma unroll
-    for (int m = 0; m < R2; ++m) {
-      const int n = t + R1 * m;
-      const float2 vm = m < R2 - KL ? v[m] : keep[(m - (R2 - KL)) * G::NT + tid];
-      float K;
-      if constexpr (OCT) {
-        const int kz = n <= H ? n : N - n;
-        const float *kb = uni(kern + ((int64_t)d * H1 * H1 + ky) * H1, z0);
-        K = *boff(kb, (uint32_t)((kz * H1 * H1 + kx) * 4));
-        if (d == 1) K *= sy;
-        if (d == 2 && n > H) K = -K;
-      } else {
-        const float *kb = uni(kern + ((int64_t)d * N * N + y) * NH, z0);
-        K = *boff(kb, (uint32_t)((n * N * NH + kx) * 4));
+      for (int m = 0; m < R2; ++m) v[m] = *boff(uni(src + (int64_t)(R1 * m) * stride, s0), lo);
+      blk_fft<N, -1>(v, c, t, S, twl);
+#pragma unroll
+      for (int m = R2 - KL; m < R2; ++m) keep[(m - (R2 - KL)) * G::NT + tid] = v[m];
+      const int ky = y <= H ? y : N - y;
+      const float sy = y <= H ? 1.f : -1.f;
+#pragma unroll 1
+      for (int d = 0; d < 3; ++d) {
+        const int z0 = opaque_zero();
+        float2 w[R2];
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          const int n = t + R1 * m;
+          const float2 vm = m < R2 - KL ? v[m] : keep[(m - (R2 - KL)) * G::NT + tid];
+          float K;
+          if constexpr (OCT) {
+            const int kz = n <= H ? n : N - n;
+            const float *kb = uni(kern + ((int64_t)d * H1 * H1 + ky) * H1, z0);
+            K = *boff(kb, (uint32_t)((kz * H1 * H1 + kx) * 4));
+            if (d == 1) K *= sy;
+            if (d == 2 && n > H) K = -K;
+          } else {
+            const float *kb = uni(kern + ((int64_t)d * N * N + y) * NH, z0);
+            K = *boff(kb, (uint32_t)((n * N * NH + kx) * 4));
+          }
+          /* rho_f(::2)=-crho(2::2)*K, rho_f(2::2)=crho(::2)*K  (W/pm.f90:119-120) */
+          w[m] = make_float2((-vm.y) * K, vm.x * K);
+        }
+        blk_fft<N, +1>(w, c, t, S, twl);
+        if (ok) {
+          float2 *dst = outd + d * plane + (int64_t)y * SP;
+          const int s1 = opaque_zero();
+#pragma unroll
+          for (int m = 0; m < R2; ++m) *boff(uni(dst + (int64_t)(R1 * m) * stride, s1), lo) = w[m];
+        }
       }
-      /* rho_f(::2)=-crho(2::2)*K, rho_f(2::2)=crho(::2)*K  (W/pm.f90:119-120) */
-      w[m] = make_float2((-vm.y) * K, vm.x * K);
-    }
-    blk_fft<N, +1>(w, c, t, S, twl);
-    if (ok) {
-      float2 *dst = outd + d * plane + (int64_t)y * SP;
-      const int s1 = opaque_zero();
-#pragma unroll
-      for (int m = 0; m < R2; ++m) *boff(uni(dst + (int64_t)(R1 * m) * stride, s1), lo) = w[m];
     }
   }
 }
@@ -605,7 +617,9 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
                                                 unsigned int *__restrict__ f2max) {
   /* half the rows of the single-component kernels: three components in LDS
      at 2 rows per workgroup keep 6 workgroups per CU, which measures faster
-     than 4 rows at 3 workgroups per CU despite idle threads in the passes */
+     than 4 rows at 3 workgroups per CU despite idle threads in the passes,
+     and faster than 4 rows one component at a time (every third float
+     stored per pass: 11.4 against 8.7 ms at 1024^3) */
   constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = L + 1;
   __shared__ float2 sm[3 * B * LP];
   __shared__ float red[FFT_T];
@@ -774,7 +788,8 @@ template <int N> struct Launch {
   static int green(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int oct, float2 *work3) {
     constexpr int NH = N / 2 + 1;
     if constexpr (RC<N>::ok) {
-      dim3 g((NH + RC<N>::CB - 1) / RC<N>::CB, N);
+      /* pairs of column blocks x the line pairs (y, N-y), y = 0..N/2 */
+      dim3 g(((NH + RC<N>::CB - 1) / RC<N>::CB + 1) / 2, N / 2 + 1);
       if (oct)
         hipLaunchKernelGGL((k_green_r<N, true>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw);
       else
