@@ -163,6 +163,19 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, int tl, const float2 
   constexpr int NB = L / R;
   constexpr int PER = (NB + TL - 1) / TL;
   float2 v[PER][R];
+  /* the twiddles do not depend on the data: their (global, cached) loads
+     are issued before the LDS reads and the barrier so their latency
+     overlaps them */
+  float2 wt[PER][R];
+  const int twmul = nt_tab / (NS * R);
+  if constexpr (NS > 1) {
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+      const int k = (tl + b * TL) % NS;
+#pragma unroll
+      for (int q = 1; q < R; ++q) wt[b][q] = tw[k * q * twmul];
+    }
+  }
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
     const int j = tl + b * TL;
@@ -172,16 +185,15 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, int tl, const float2 
     }
   }
   __syncthreads();
-  const int twmul = nt_tab / (NS * R);
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
     const int j = tl + b * TL;
     if (PER * TL == NB || j < NB) {
       const int k = j % NS;
-      if (NS > 1) {
+      if constexpr (NS > 1) {
 #pragma unroll
         for (int q = 1; q < R; ++q) {
-          float2 w = tw[k * q * twmul];
+          float2 w = wt[b][q];
           if (DIR > 0) w.y = -w.y;
           v[b][q] = cmul(v[b][q], w);
         }
@@ -502,9 +514,19 @@ __global__ __launch_bounds__(FFT_T) void k_r2c_x(const float *__restrict__ in, f
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int64_t row0 = (int64_t)blockIdx.x * B;
   const float2 *in2 = reinterpret_cast<const float2 *>(in);
-  for (int idx = tid; idx < B * L; idx += FFT_T) {
-    const int l = idx / L, m = idx % L;
-    sm[l * LP + m] = (row0 + l < nrows) ? in2[(row0 + l) * L + m] : make_float2(0.f, 0.f);
+  /* every load of the rows in flight before the first LDS store */
+  constexpr int IT = (B * L + FFT_T - 1) / FFT_T;
+  float2 ld[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * FFT_T, l = idx / L, m = idx % L;
+    ld[i] = ((B * L) % FFT_T == 0 || idx < B * L) && row0 + l < nrows ? in2[(row0 + l) * L + m]
+                                                                     : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * FFT_T, l = idx / L, m = idx % L;
+    if ((B * L) % FFT_T == 0 || idx < B * L) sm[l * LP + m] = ld[i];
   }
   __syncthreads();
   line_fft<L, -1, TL>(sm + line * LP, tl, tw, N);
@@ -532,9 +554,18 @@ __global__ __launch_bounds__(FFT_T) void k_c2c_s(float2 *__restrict__ a, const f
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int col0 = blockIdx.x * B;
   float2 *base = a + (int64_t)blockIdx.y * ostride + col0;
-  for (int idx = tid; idx < B * N; idx += FFT_T) {
-    const int l = idx % B, n = idx / B;
-    sm[l * NP + n] = (col0 + l < ncol) ? base[n * stride + l] : make_float2(0.f, 0.f);
+  /* every load in flight before the first LDS store */
+  constexpr int IT = (B * N + FFT_T - 1) / FFT_T;
+  float2 ld[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * FFT_T, l = idx % B, n = idx / B;
+    ld[i] = ((B * N) % FFT_T == 0 || idx < B * N) && col0 + l < ncol ? base[n * stride + l] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * FFT_T, l = idx % B, n = idx / B;
+    if ((B * N) % FFT_T == 0 || idx < B * N) sm[l * NP + n] = ld[i];
   }
   __syncthreads();
   line_fft<N, DIR, TL>(sm + line * NP, tl, tw, N);
@@ -630,19 +661,36 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int64_t row0 = (int64_t)blockIdx.x * B;
   const int64_t plane = nrows * cube_spec_pitch(N);
-  for (int d = 0; d < 3; ++d) {
-    const float2 *in = specd + d * plane;
-    /* Z[k] = E + i O, E = X[k]+conj(X[L-k]), O = (X[k]-conj(X[L-k])) W^-k */
-    for (int idx = tid; idx < B * L; idx += FFT_T) {
-      const int l = idx / L, k = idx % L;
+  /* Z[k] = E + i O, E = X[k]+conj(X[L-k]), O = (X[k]-conj(X[L-k])) W^-k;
+     every load of the three components in flight before the first store */
+  constexpr int IT = (B * L + FFT_T - 1) / FFT_T;
+  float2 xk[3][IT], xc[3][IT], wk[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * FFT_T, l = idx / L, k = idx % L;
+    const bool in = ((B * L) % FFT_T == 0 || idx < B * L) && row0 + l < nrows;
+    wk[i] = tw[in ? k : 0];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float2 *sp = specd + d * plane + (row0 + (in ? l : 0)) * cube_spec_pitch(N);
+      xk[d][i] = sp[in ? k : 0];
+      xc[d][i] = sp[in ? L - k : 0];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * FFT_T, l = idx / L, k = idx % L;
+    if ((B * L) % FFT_T != 0 && idx >= B * L) continue;
+    const bool in = row0 + l < nrows;
+    float2 w = wk[i];
+    w.y = -w.y;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
       float2 z = make_float2(0.f, 0.f);
-      if (row0 + l < nrows) {
-        const float2 xk = in[(row0 + l) * cube_spec_pitch(N) + k];
-        const float2 xc = in[(row0 + l) * cube_spec_pitch(N) + (L - k)];
-        const float2 e = make_float2(xk.x + xc.x, xk.y - xc.y);
-        float2 w = tw[k];
-        w.y = -w.y;
-        const float2 o = cmul(make_float2(xk.x - xc.x, xk.y + xc.y), w);
+      if (in) {
+        const float2 a = xk[d][i], c = xc[d][i];
+        const float2 e = make_float2(a.x + c.x, a.y - c.y);
+        const float2 o = cmul(make_float2(a.x - c.x, a.y + c.y), w);
         z = make_float2(e.x - o.y, e.y + o.x);
       }
       sm[(d * B + l) * LP + k] = z;
