@@ -138,6 +138,33 @@ template <int B> struct IntOf;
 template <> struct IntOf<1> { typedef int8_t T; };
 template <> struct IntOf<2> { typedef int16_t T; };
 
+/* inclusive prefix sum over the 64 lanes of a wave (shuffles, no LDS) */
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    v += lane >= o ? u : 0;
+  }
+  return v;
+}
+/* inclusive prefix sum over a workgroup of NW waves (one barrier; `tot`
+   gets the workgroup total; wsum is NW ints of LDS) */
+template <int NW> __device__ __forceinline__ int block_incl_scan(int v, int *wsum, int &tot) {
+  const int w = threadIdx.x >> 6;
+  v = wave_incl_scan(v);
+  if ((threadIdx.x & 63) == 63) wsum[w] = v;
+  __syncthreads();
+  int base = 0;
+  tot = 0;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    base += q < w ? wsum[q] : 0;
+    tot += wsum[q];
+  }
+  return v + base;
+}
+
 /* cube_encode_v through the code boundaries T[k] = tan((k+0.5)*pi/(nvbin-1))
    for k < kmax and T[kmax] = +inf (venc_tab, built on the host in long
    double; pi is the real(4) pi of the reference): the code of |u| is the

@@ -626,16 +626,20 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
       vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal[a]);
     }
   }
-  if (!FINE) {
-    __shared__ double red[3][KT];
-    for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = vm[a];
+  if (!FINE) { /* max over the workgroup: wave shuffles, then the waves through LDS */
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) vm[a] = fmax(vm[a], __shfl_xor(vm[a], o, 64));
+    __shared__ double red[3][KT / 64];
+    if ((threadIdx.x & 63) == 0)
+      for (int a = 0; a < 3; ++a) red[a][threadIdx.x >> 6] = vm[a];
     __syncthreads();
-    for (int o = KT / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o)
-        for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = fmax(red[a][threadIdx.x], red[a][threadIdx.x + o]);
-      __syncthreads();
+    if (threadIdx.x < 3) {
+      double m = red[threadIdx.x][0];
+      for (int w = 1; w < KT / 64; ++w) m = fmax(m, red[threadIdx.x][w]);
+      atomicMax(vmax_bits + threadIdx.x, (unsigned long long)__double_as_longlong(m));
     }
-    if (threadIdx.x < 3) atomicMax(vmax_bits + threadIdx.x, (unsigned long long)__double_as_longlong(red[threadIdx.x][0]));
   }
 }
 

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
   uint64_t *pos = smem64;                                            /* packed positions (maxs) */
   uint16_t *cur = reinterpret_cast<uint16_t *>(smem64 + maxs) + 1; /* per fine cell, cur[-1] = 0 */
-  __shared__ int part[PPB_T];
+  __shared__ int part[PPB_T / 64];
   __shared__ int64_t cst[16][5];
   __shared__ int rowoff[17];
   const int tid = threadIdx.x, nt = g.nt, nb = nt / 4, nnt = g.nnt;
@@ -224,17 +224,10 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   const int i0 = tid * chunk, i1 = min(i0 + chunk, nreg);
   int s = 0;
   for (int i = i0; i < i1; ++i) s += cur[i];
-  part[tid] = s;
-  __syncthreads();
-  for (int o = 1; o < PPB_T; o <<= 1) {
-    const int v = tid >= o ? part[tid - o] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  const int nstage = part[PPB_T - 1];
+  int nstage;
+  const int incl = block_incl_scan<PPB_T / 64>(s, part, nstage);
   const bool staged = nstage <= maxs; /* uniform over the block */
-  s = part[tid] - s;
+  s = incl - s;
   for (int i = i0; i < i1; ++i) {
     const int cnt = cur[i];
     cur[i] = (uint16_t)(staged ? s : 0);
@@ -377,23 +370,28 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     f2m = f2 > f2m ? f2 : f2m;
     ++ntest;
   }
-  /* block reduction through the (now free) staging area */
-  __syncthreads();
-  double *red = reinterpret_cast<double *>(smem64);
-  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(smem64) + PPB_T;
-  red[tid] = f2m;
-  cnt[tid] = ntest;
-  __syncthreads();
-  for (int o = PPB_T / 2; o > 0; o >>= 1) {
-    if (tid < o) {
-      red[tid] = fmax(red[tid], red[tid + o]);
-      cnt[tid] += cnt[tid + o];
-    }
-    __syncthreads();
+  /* max |f|^2 and the particle count: wave shuffles, the waves through
+     LDS, one global atomic pair per workgroup (max and sum do not depend
+     on the order) */
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    f2m = fmax(f2m, __shfl_xor(f2m, o, 64));
+    ntest += __shfl_xor(ntest, o, 64);
   }
+  __shared__ double wf2[PPB_T / 64];
+  __shared__ unsigned long long wn[PPB_T / 64];
+  if ((tid & 63) == 0) {
+    wf2[tid >> 6] = f2m;
+    wn[tid >> 6] = ntest;
+  }
+  __syncthreads();
   if (tid == 0) {
-    atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
-    atomicAdd(out + 1, cnt[0]);
+    for (int w = 1; w < PPB_T / 64; ++w) {
+      f2m = fmax(f2m, wf2[w]);
+      ntest += wn[w];
+    }
+    atomicMax(out, (unsigned long long)__double_as_longlong(f2m));
+    atomicAdd(out + 1, ntest);
   }
 }
 
