@@ -35,8 +35,11 @@ static void free_all(cube_ctx *c) {
     if (p) (void)hipFree(p);
   cube_fft_destroy(c->plan_f);
   cube_fft_destroy(c->plan_c);
-  if (c->tev[0]) (void)hipEventDestroy(c->tev[0]);
-  if (c->tev[1]) (void)hipEventDestroy(c->tev[1]);
+  for (int k = 0; k < 2 * CUBE_TEV; ++k)
+    if (c->tev[k]) (void)hipEventDestroy(c->tev[k]);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -74,6 +77,13 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
     return CUBE_ERR_HIP;
   }
   if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return CUBE_ERR_HIP;
+  }
+  if (hipStreamCreate(&c->side) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    free_all(c);
     delete c;
     return CUBE_ERR_HIP;
   }
@@ -137,7 +147,7 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->specd_c, 3 * nspec_c);
   st |= dalloc(&c->force_c, 3 * nc3);
   st |= dalloc(&c->kern_c, 3 * (c->nc / 2 + 1) * c->nc * c->nc); /* unpadded [d][kz][ky][kx] */
-  st |= dalloc(&c->dred, 4096);
+  st |= dalloc(&c->dred, 8192);
   st |= dalloc(&c->ured, 64);
   st |= dalloc(&c->ired, 4096);
   st |= dalloc(&c->scanbuf, c->ncells / 1024 + 4);
@@ -214,16 +224,17 @@ extern "C" double cube_host_atan(double x) { return cube_atan(x); }
 extern "C" int cube_profile(cube_ctx *c, int on) {
   if (!c) return CUBE_ERR_INVALID;
   CUBE_CHECK(hipSetDevice(c->device));
-  if (on && !c->tev[0]) {
-    CUBE_CHECK(hipEventCreate(&c->tev[0]));
-    CUBE_CHECK(hipEventCreate(&c->tev[1]));
-  }
+  if (on && !c->tev[0])
+    for (int k = 0; k < 2 * CUBE_TEV; ++k) CUBE_CHECK(hipEventCreate(&c->tev[k]));
+  if (!on) cube_timers_resolve(c);
   c->profiling = on ? 1 : 0;
   return CUBE_OK;
 }
 
 extern "C" int cube_timers(cube_ctx *c, double *ms, int64_t *calls, int reset) {
   if (!c) return CUBE_ERR_INVALID;
+  CUBE_CHECK(hipSetDevice(c->device));
+  cube_timers_resolve(c);
   for (int i = 0; i < CUBE_NTIMERS; ++i) {
     if (ms) ms[i] = c->tms[i];
     if (calls) calls[i] = c->tcalls[i];
@@ -369,14 +380,14 @@ extern "C" int cube_pm_fine_force(cube_ctx *c, float *f2) {
   if (st) return st;
   tend(c, CUBE_T_FINE_FFT);
   return cube_fft_green_c2r_3d(c, c->plan_f, c->spec_f, c->kern_f, 1, c->specd_f, c->force_f, (float)c->nf, f2,
-                               CUBE_T_FINE_GREEN);
+                               CUBE_T_FINE_GREEN, (unsigned *)c->ured);
 }
 extern "C" int cube_pm_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new) {
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
   tbeg(c);
-  const int st = cube_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new);
+  const int st = cube_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new, 1);
   tend(c, CUBE_T_FINE_KICK);
   return st;
 }
@@ -396,7 +407,8 @@ extern "C" int cube_pm_coarse_force(cube_ctx *c, float *f2) {
   tbeg(c);
   int st = cube_fft_r2c_3d(c, c->plan_c, c->rho_c, c->spec_c);
   if (st) return st;
-  st = cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc, f2, -1);
+  st = cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc, f2, -1,
+                             (unsigned *)c->ured + 1);
   tend(c, CUBE_T_COARSE_FORCE);
   return st;
 }
@@ -418,13 +430,57 @@ extern "C" int cube_particle_mesh(cube_ctx *c, float a_mid, float dt, float sigm
   cube_pm_out o;
   memset(&o, 0, sizeof(o));
   int st;
-  if ((st = cube_pm_fine_deposit(c, mass_p, &o.testrhof))) return st;
-  if ((st = cube_pm_fine_force(c, &o.f2_max_fine))) return st;
-  if ((st = cube_pm_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new))) return st;
+  CUBE_CHECK(hipSetDevice(c->device));
+  /* The coarse deposit and force read only the positions, so they run on
+     the side stream beside the fine deposit and force; the two kicks,
+     which both rewrite the velocities, stay in the reference's order on
+     the main stream (W/pm.f90:2-415).  Scalars are read back at the end. */
+  CUBE_CHECK(hipEventRecord(c->ev_fork, c->stream));
+  CUBE_CHECK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  tbeg(c);
+  if ((st = cube_fine_deposit(c, mass_p, nullptr))) return st;
+  tend(c, CUBE_T_FINE_DEPOSIT);
+  if ((st = cube_pm_fine_force(c, nullptr))) return st;
+  {
+    hipStream_t main = c->stream;
+    c->stream = c->side;
+    tbeg(c);
+    st = cube_coarse_deposit(c, mass_p, nullptr);
+    tend(c, CUBE_T_COARSE_DEPOSIT);
+    if (!st) {
+      tbeg(c);
+      st = cube_fft_r2c_3d(c, c->plan_c, c->rho_c, c->spec_c);
+      if (!st)
+        st = cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc,
+                                   nullptr, -1, (unsigned *)c->ured + 1);
+      tend(c, CUBE_T_COARSE_FORCE);
+    }
+    const hipError_t e = hipEventRecord(c->ev_join, c->side);
+    c->stream = main;
+    if (st) return st;
+    if (e != hipSuccess) return CUBE_ERR_HIP;
+  }
+  tbeg(c);
+  st = cube_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new, 0);
+  tend(c, CUBE_T_FINE_KICK);
+  if (st) return st;
   sigma_vi = sigma_vi_new; /* W/pm.f90:193 */
-  if ((st = cube_pm_coarse_deposit(c, mass_p, &o.testrhoc))) return st;
-  if ((st = cube_pm_coarse_force(c, &o.f2_max_coarse))) return st;
-  if ((st = cube_pm_coarse_kick(c, a_mid, dt, sigma_vi, o.vmax))) return st;
+  CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  if ((st = cube_pm_coarse_kick(c, a_mid, dt, sigma_vi, o.vmax))) return st; /* synchronises */
+  {
+    double sums[2];
+    unsigned f2[2];
+    CUBE_CHECK(hipMemcpyAsync(&sums[0], c->dred + CUBE_DRED_FINE_SUM, sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    CUBE_CHECK(hipMemcpyAsync(&sums[1], c->dred + CUBE_DRED_COARSE_SUM, sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    CUBE_CHECK(hipMemcpyAsync(f2, c->ured, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    CUBE_CHECK(hipStreamSynchronize(c->stream));
+    o.testrhof = sums[0];
+    o.testrhoc = sums[1];
+    memcpy(&o.f2_max_fine, &f2[0], sizeof(float));
+    memcpy(&o.f2_max_coarse, &f2[1], sizeof(float));
+  }
   /* W/pm.f90:395-398 in real(4) */
   const float GG = 1.0f / 6.0f / CUBE_PI_F;
   o.dt_fine = sqrtf(1.0f / (sqrtf(o.f2_max_fine) * a_mid * GG));

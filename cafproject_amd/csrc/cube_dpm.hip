@@ -257,7 +257,7 @@ static int force_of(cube_dpm *d, const float *x) {
   LAUNCH(k_dpm_deposit, x, d->np, nf, d->rho);
   if ((st = cube_fft_r2c_3d(&d->c, d->plan_f, d->rho, d->spec_f))) return st;
   return cube_fft_green_c2r_3d(&d->c, d->plan_f, d->spec_f, d->kern, 0, d->work3_f, d->force, (float)nf, nullptr,
-                               -1);
+                               -1, (unsigned *)d->c.ured);
 }
 
 /* one KDK step on (x, pm) */

@@ -912,8 +912,9 @@ int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out) {
 /* spec: output of cube_fft_r2c_3d (x and y transformed).  Produces the
    three force components (3 floats per cell) and max |F|^2. */
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
-                          float2 *work3, float *force3, float div, float *f2max, int t0) {
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, (unsigned *)c->ured, 1);
+                          float2 *work3, float *force3, float div, float *f2max, int t0, unsigned *f2dev) {
+  /* max |F|^2 as float bits in the device word f2dev */
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, f2dev, 1);
   CUBE_LAUNCH_CHECK();
   const int64_t plane = (int64_t)cube_spec_pitch(p->n) * p->n * p->n;
   switch (p->n) {
@@ -925,7 +926,7 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
     if (t0 >= 0) { tend(c, t0); tbeg(c); }                                        \
     if ((st = Launch<N>::c2c_y<+1>(c, p, work3, 3))) return st;                   \
     if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                    \
-    if ((st = Launch<N>::c2r3(c, p, work3, force3, div, (unsigned *)c->ured))) return st; \
+    if ((st = Launch<N>::c2r3(c, p, work3, force3, div, f2dev))) return st;       \
     if (t0 >= 0) tend(c, t0 + 2);                                                 \
     break;                                                                        \
   }
@@ -936,7 +937,7 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
   }
   if (!f2max) return CUBE_OK; /* no readback: stays asynchronous (and capturable) */
   unsigned bits = 0;
-  CUBE_CHECK(hipMemcpyAsync(&bits, c->ured, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  CUBE_CHECK(hipMemcpyAsync(&bits, f2dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
   CUBE_CHECK(hipStreamSynchronize(c->stream));
   float f;
   memcpy(&f, &bits, sizeof(f));

@@ -38,6 +38,8 @@
 
 struct FFTPlan;  /* cube_fft.hip */
 
+#define CUBE_TEV 64 /* stage timer event pairs held before they are resolved */
+
 struct cube_ctx {
   cube_params p;
   int device;
@@ -81,7 +83,8 @@ struct cube_ctx {
   float *kern_c;            /* 3*nc*nc*(nc/2+1) */
   int has_kernels;
   /* small device scratch for reductions */
-  double *dred;             /* 1024 doubles */
+  double *dred;             /* 8192 doubles: [0,4096) reductions of work on `stream`,
+                               [4096,8192) of work on `side` */
   uint64_t *ured;           /* 64 words */
   int64_t *ired;            /* 4096 */
   int64_t *scanbuf;         /* ncells/1024 + 2 block sums of the scan */
@@ -89,25 +92,42 @@ struct cube_ctx {
   /* host-side persistent state */
   float pp_ftemp;           /* running max of W/ext_pp_force.f90:123 */
   FFTPlan *plan_f, *plan_c;
-  /* stage timers */
+  /* the coarse PM pipeline runs on `side` beside the fine one on `stream` */
+  hipStream_t side;
+  hipEvent_t ev_fork, ev_join;
+  /* stage timers: event pairs recorded without waiting, resolved when the
+     ring is full or the timers are read */
   int profiling;
-  hipEvent_t tev[2];
+  hipEvent_t tev[2 * CUBE_TEV];
+  int tid[CUBE_TEV];
+  int tused, topen;
   double tms[CUBE_NTIMERS];
   int64_t tcalls[CUBE_NTIMERS];
 };
 
-/* stage timing on the context stream (no-ops unless profiling) */
+/* add the finished event pairs to the stage totals (waits for them) */
+static inline void cube_timers_resolve(cube_ctx *c) {
+  for (int k = 0; k < c->tused; ++k) {
+    (void)hipEventSynchronize(c->tev[2 * k + 1]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->tev[2 * k], c->tev[2 * k + 1]);
+    c->tms[c->tid[k]] += ms;
+    c->tcalls[c->tid[k]] += 1;
+  }
+  c->tused = 0;
+}
+/* stage timing on the stream the stage runs on (no-ops unless profiling);
+   host code opens at most one stage at a time */
 static inline void tbeg(cube_ctx *c) {
-  if (c->profiling) (void)hipEventRecord(c->tev[0], c->stream);
+  if (!c->profiling) return;
+  if (c->tused == CUBE_TEV) cube_timers_resolve(c);
+  c->topen = c->tused++;
+  (void)hipEventRecord(c->tev[2 * c->topen], c->stream);
 }
 static inline void tend(cube_ctx *c, int id) {
   if (!c->profiling) return;
-  (void)hipEventRecord(c->tev[1], c->stream);
-  (void)hipEventSynchronize(c->tev[1]);
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, c->tev[0], c->tev[1]);
-  c->tms[id] += ms;
-  c->tcalls[id] += 1;
+  c->tid[c->topen] = id;
+  (void)hipEventRecord(c->tev[2 * c->topen + 1], c->stream);
 }
 
 /* --- common device helpers --- */
@@ -203,12 +223,16 @@ int cube_fft_create(cube_ctx *c, int n, FFTPlan **out);
 void cube_fft_destroy(FFTPlan *p);
 int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
-                          float2 *work3, float *force3, float scale_div, float *f2max, int timer0);
+                          float2 *work3, float *force3, float scale_div, float *f2max, int timer0,
+                          unsigned *f2dev);
 int cube_fft_forward_full(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out);
 
 int cube_fine_deposit(cube_ctx *c, float mass_p, double *testrhof);
-int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new);
+/* device slots (in dred) of the deposits' mesh sums */
+#define CUBE_DRED_FINE_SUM 4000
+#define CUBE_DRED_COARSE_SUM (4096 + 4000)
+int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new, int sync);
 int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc);
 int cube_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax);
 int cube_green_init(cube_ctx *c, const float *wfxyzf, const float *wfxyzc);

@@ -457,10 +457,11 @@ template <int IX> static int fine_dep(cube_ctx *c, float mass, double *t) {
   hipLaunchKernelGGL(k_fine_deposit_cc<IX>, dim3(DEP_BLOCKS), dim3(FD_T), 0, c->stream,
                      (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_f, c->dred);
   CUBE_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, c->dred, DEP_BLOCKS, c->dred + 4000);
+  hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, c->dred, DEP_BLOCKS,
+                     c->dred + CUBE_DRED_FINE_SUM);
   CUBE_LAUNCH_CHECK();
   if (t) {
-    CUBE_CHECK(hipMemcpyAsync(t, c->dred + 4000, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CUBE_CHECK(hipMemcpyAsync(t, c->dred + CUBE_DRED_FINE_SUM, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CUBE_CHECK(hipStreamSynchronize(c->stream));
   }
   return CUBE_OK;
@@ -469,17 +470,21 @@ int cube_fine_deposit(cube_ctx *c, float mass_p, double *testrhof) {
   return c->p.izipx == 1 ? fine_dep<1>(c, mass_p, testrhof) : fine_dep<2>(c, mass_p, testrhof);
 }
 template <int IX> static int coarse_dep(cube_ctx *c, float mass, double *t) {
+  /* partial sums in the upper half of dred, apart from the fine deposit's,
+     so that the two deposits may run at the same time */
+  double *part = c->dred + 4096;
   if (c->nt % 2 == 0)
     hipLaunchKernelGGL(k_coarse_deposit_b2<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
-                       (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, c->dred);
+                       (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, part);
   else
     hipLaunchKernelGGL(k_coarse_deposit<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
-                       (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, c->dred);
+                       (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, part);
   CUBE_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, c->dred, DEP_BLOCKS, c->dred + 4000);
+  hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, part, DEP_BLOCKS,
+                     c->dred + CUBE_DRED_COARSE_SUM);
   CUBE_LAUNCH_CHECK();
   if (t) {
-    CUBE_CHECK(hipMemcpyAsync(t, c->dred + 4000, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CUBE_CHECK(hipMemcpyAsync(t, c->dred + CUBE_DRED_COARSE_SUM, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CUBE_CHECK(hipStreamSynchronize(c->stream));
   }
   return CUBE_OK;
@@ -663,12 +668,12 @@ static int kick(cube_ctx *c, const float *force, float a_mid, float dt, double v
 }
 
 /* fine kick, W/pm.f90:129-159: decode with sigma_vi, encode with sigma_vi_new */
-int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new) {
+int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new, int sync) {
   int st = cube_vdec_table(c, c->vdec0, sigma_vi);
   if (st) return st;
   st = kick<true>(c, c->force_f, a_mid, dt, cube_vconst(sigma_vi_new), nullptr);
   if (st) return st;
-  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  if (sync) CUBE_CHECK(hipStreamSynchronize(c->stream));
   return CUBE_OK;
 }
 
