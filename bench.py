@@ -108,6 +108,10 @@ MAIN_KERNEL = {
     "fine_fft": "k_r2c_x+k_c2c_r", "fine_inverse": "k_green_r+k_c2c_r+k_c2r_x3",
     "coarse_force": "coarse fft kernels",
 }
+# stages whose kernels share the GPU with another stream's (particle_mesh runs
+# the coarse deposit and force on a side stream beside the fine deposit)
+CONCURRENT = {"fine_deposit": "coarse_deposit+coarse_force", "coarse_deposit": "fine_deposit",
+              "coarse_force": "fine_deposit"}
 # kernel-name prefixes in the rocprofv3 PMC traffic table, per single-kernel stage
 PMC_NAMES = {"pp": ["k_pp_blk<"], "fine_deposit": ["k_fine_deposit_cc<"], "fine_kick": ["k_kick<"],
              "coarse_deposit": ["k_coarse_deposit_b2<"], "coarse_kick": ["k_kick<"]}
@@ -285,6 +289,7 @@ def main():
                    "particles": np_, "nc": nc, "nnt": nnt, "nf": 4 * nc, "pp": pp_on,
                    "izipx": izipx, "izipv": izipv},
         "roofline": {"bound": "hbm", "kernel": MAIN_KERNEL[dom], "stage": dom,
+                     "concurrent_with": CONCURRENT.get(dom),
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": pmc_traffic(dom, nc),
                      "traffic_source": os.path.relpath(PMC_TABLE, ROOT) if pmc_traffic(dom, nc) else None,
