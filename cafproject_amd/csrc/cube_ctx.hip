@@ -39,7 +39,9 @@ static void free_all(cube_ctx *c) {
     if (c->tev[k]) (void)hipEventDestroy(c->tev[k]);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
   if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->side2) (void)hipStreamDestroy(c->side2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -80,9 +82,10 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
     delete c;
     return CUBE_ERR_HIP;
   }
-  if (hipStreamCreate(&c->side) != hipSuccess ||
+  if (hipStreamCreate(&c->side) != hipSuccess || hipStreamCreate(&c->side2) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming) != hipSuccess) {
     free_all(c);
     delete c;
     return CUBE_ERR_HIP;
@@ -249,6 +252,7 @@ extern "C" int cube_timers(cube_ctx *c, double *ms, int64_t *calls, int reset) {
 /* ---- state transfer ---------------------------------------------------- */
 extern "C" int cube_set_state(cube_ctx *c, int64_t nplocal, int64_t npglobal, const void *xp, const void *vp,
                               const int32_t *pid, const int32_t *rhoc, const float *vfield) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c || !xp || !vp || !rhoc || !vfield || nplocal < 0) return CUBE_ERR_INVALID;
   if (nplocal > c->cap) return CUBE_ERR_IMAGE_OVERFLOW;
   CUBE_CHECK(hipSetDevice(c->device));
@@ -293,6 +297,7 @@ extern "C" int cube_get_state(cube_ctx *c, int64_t *nplocal, void *xp, void *vp,
 
 /* ---- kernels ----------------------------------------------------------- */
 extern "C" int cube_init_kernels(cube_ctx *c, const float *wfxyzf, const float *wfxyzc) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c || !wfxyzf || !wfxyzc) return CUBE_ERR_INVALID;
   CUBE_CHECK(hipSetDevice(c->device));
   int st = cube_green_init(c, wfxyzf, wfxyzc);
@@ -305,6 +310,7 @@ extern "C" int cube_init_kernels(cube_ctx *c, const float *wfxyzf, const float *
 /* ---- the step ------------------------------------------------------------ */
 extern "C" int cube_update_x(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi,
                              cube_vstats *out) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
@@ -339,6 +345,44 @@ extern "C" int cube_buffer_v(cube_ctx *c) {
   return c->has_state ? CUBE_OK : CUBE_ERR_STATE;
 }
 
+/* The fine deposit and force on stream fs and the coarse deposit and force
+   on stream cs, both after the work already queued on the context stream;
+   ev_join and ev_join2 mark their ends.  Nothing is read back: the mesh sums
+   stay in dred and max |F|^2 in ured words 0 (fine) and 1 (coarse).  The
+   two pipelines read only the positions and write disjoint buffers. */
+static int pm_forces(cube_ctx *c, float mass_p, hipStream_t fs, hipStream_t cs) {
+  const hipStream_t main = c->stream;
+  CUBE_CHECK(hipEventRecord(c->ev_fork, main));
+  int st = CUBE_OK;
+  c->stream = fs;
+  if (fs != main && hipStreamWaitEvent(fs, c->ev_fork, 0) != hipSuccess) st = CUBE_ERR_HIP;
+  if (!st) {
+    tbeg(c);
+    st = cube_fine_deposit(c, mass_p, nullptr);
+    tend(c, CUBE_T_FINE_DEPOSIT);
+  }
+  if (!st) st = cube_pm_fine_force(c, nullptr);
+  if (!st && hipEventRecord(c->ev_join, fs) != hipSuccess) st = CUBE_ERR_HIP;
+  c->stream = cs;
+  if (!st && hipStreamWaitEvent(cs, c->ev_fork, 0) != hipSuccess) st = CUBE_ERR_HIP;
+  if (!st) {
+    tbeg(c);
+    st = cube_coarse_deposit(c, mass_p, nullptr);
+    tend(c, CUBE_T_COARSE_DEPOSIT);
+  }
+  if (!st) {
+    tbeg(c);
+    st = cube_fft_r2c_3d(c, c->plan_c, c->rho_c, c->spec_c);
+    if (!st)
+      st = cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc,
+                                 nullptr, -1, (unsigned *)c->ured + 1);
+    tend(c, CUBE_T_COARSE_FORCE);
+  }
+  if (!st && hipEventRecord(c->ev_join2, cs) != hipSuccess) st = CUBE_ERR_HIP;
+  c->stream = main;
+  return st;
+}
+
 extern "C" int cube_pp_force(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, float *f2_max_pp) {
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
@@ -353,16 +397,35 @@ extern "C" int cube_pp_force(cube_ctx *c, float a_mid, float dt, float sigma_vi,
   free(sums);
   if (st) return st;
   if (mx > c->np_pp_max) return CUBE_ERR_PP_OVERFLOW;
+  /* The PM forces of this step depend only on the positions (final after
+     update_x) and on mass_p, not on the velocities PP updates: they are
+     queued on the side streams first and run beside PP.  The call still
+     returns with all its work done; particle_mesh uses them when nothing
+     has changed the state in between and mass_p agrees. */
+  c->pm_ahead = 0;
+  if (c->has_kernels) {
+    st = pm_forces(c, mass_p, c->side, c->side2);
+    if (st) return st;
+  }
   int64_t itest1 = 0;
   tbeg(c);
   st = cube_pp(c, a_mid, dt, sigma_vi, mass_p, f2_max_pp, &itest1);
   tend(c, CUBE_T_PP);
+  if (c->has_kernels) {
+    CUBE_CHECK(hipEventSynchronize(c->ev_join));
+    CUBE_CHECK(hipEventSynchronize(c->ev_join2));
+  }
   if (st) return st;
   if (itest1 != c->nplocal) return CUBE_ERR_COUNT;
+  if (c->has_kernels) {
+    c->pm_ahead = 1;
+    c->pm_mass = mass_p;
+  }
   return CUBE_OK;
 }
 
 extern "C" int cube_pm_fine_deposit(cube_ctx *c, float mass_p, double *testrhof) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
@@ -372,6 +435,7 @@ extern "C" int cube_pm_fine_deposit(cube_ctx *c, float mass_p, double *testrhof)
   return st;
 }
 extern "C" int cube_pm_fine_force(cube_ctx *c, float *f2) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_kernels) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
@@ -383,6 +447,7 @@ extern "C" int cube_pm_fine_force(cube_ctx *c, float *f2) {
                                CUBE_T_FINE_GREEN, (unsigned *)c->ured);
 }
 extern "C" int cube_pm_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
@@ -392,6 +457,7 @@ extern "C" int cube_pm_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma
   return st;
 }
 extern "C" int cube_pm_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
@@ -401,6 +467,7 @@ extern "C" int cube_pm_coarse_deposit(cube_ctx *c, float mass_p, double *testrho
   return st;
 }
 extern "C" int cube_pm_coarse_force(cube_ctx *c, float *f2) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_kernels) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
@@ -413,6 +480,7 @@ extern "C" int cube_pm_coarse_force(cube_ctx *c, float *f2) {
   return st;
 }
 extern "C" int cube_pm_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   CUBE_CHECK(hipSetDevice(c->device));
@@ -431,41 +499,21 @@ extern "C" int cube_particle_mesh(cube_ctx *c, float a_mid, float dt, float sigm
   memset(&o, 0, sizeof(o));
   int st;
   CUBE_CHECK(hipSetDevice(c->device));
-  /* The coarse deposit and force read only the positions, so they run on
-     the side stream beside the fine deposit and force; the two kicks,
-     which both rewrite the velocities, stay in the reference's order on
-     the main stream (W/pm.f90:2-415).  Scalars are read back at the end. */
-  CUBE_CHECK(hipEventRecord(c->ev_fork, c->stream));
-  CUBE_CHECK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-  tbeg(c);
-  if ((st = cube_fine_deposit(c, mass_p, nullptr))) return st;
-  tend(c, CUBE_T_FINE_DEPOSIT);
-  if ((st = cube_pm_fine_force(c, nullptr))) return st;
-  {
-    hipStream_t main = c->stream;
-    c->stream = c->side;
-    tbeg(c);
-    st = cube_coarse_deposit(c, mass_p, nullptr);
-    tend(c, CUBE_T_COARSE_DEPOSIT);
-    if (!st) {
-      tbeg(c);
-      st = cube_fft_r2c_3d(c, c->plan_c, c->rho_c, c->spec_c);
-      if (!st)
-        st = cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc,
-                                   nullptr, -1, (unsigned *)c->ured + 1);
-      tend(c, CUBE_T_COARSE_FORCE);
-    }
-    const hipError_t e = hipEventRecord(c->ev_join, c->side);
-    c->stream = main;
-    if (st) return st;
-    if (e != hipSuccess) return CUBE_ERR_HIP;
-  }
+  /* The coarse deposit and force read only the positions, so they run on a
+     side stream beside the fine deposit and force (unless ext_pp_force
+     already formed both beside PP); the two kicks, which both rewrite the
+     velocities, stay in the reference's order on the main stream
+     (W/pm.f90:2-415).  Scalars are read back at the end. */
+  const int ahead = c->pm_ahead && c->pm_mass == mass_p;
+  c->pm_ahead = 0;
+  if (!ahead && (st = pm_forces(c, mass_p, c->stream, c->side))) return st;
+  CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   tbeg(c);
   st = cube_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new, 0);
   tend(c, CUBE_T_FINE_KICK);
   if (st) return st;
   sigma_vi = sigma_vi_new; /* W/pm.f90:193 */
-  CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
   if ((st = cube_pm_coarse_kick(c, a_mid, dt, sigma_vi, o.vmax))) return st; /* synchronises */
   {
     double sums[2];
@@ -519,6 +567,7 @@ extern "C" int cube_get_field(cube_ctx *c, int which, void *host, size_t bytes) 
   return CUBE_OK;
 }
 extern "C" int cube_set_field(cube_ctx *c, int which, const void *host, size_t bytes) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c || !host) return CUBE_ERR_INVALID;
   void *p;
   size_t n;

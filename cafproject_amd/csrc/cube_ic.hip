@@ -244,6 +244,7 @@ static int state_from_device_xv(cube_ctx *c, const double *x, const float *v, in
 
 extern "C" int cube_set_state_xv(cube_ctx *c, int64_t n, const double *x, const float *v, float sigma_vi,
                                  float *sigma_out) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c || !x || !v || n < 0) return CUBE_ERR_INVALID;
   CUBE_CHECK(hipSetDevice(c->device));
   /* stage through the (idle) fine-mesh buffers */
@@ -259,6 +260,7 @@ extern "C" int cube_set_state_xv(cube_ctx *c, int64_t n, const double *x, const 
 }
 
 extern "C" int cube_ic_zeldovich(cube_ctx *c, const cube_ic_params *ip, float *sigma_out) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c || !ip || ip->ntab < 2 || !ip->lnp) return CUBE_ERR_INVALID;
   CUBE_CHECK(hipSetDevice(c->device));
   const int ng = (int)(c->nc * c->p.np_nc);

@@ -92,9 +92,13 @@ struct cube_ctx {
   /* host-side persistent state */
   float pp_ftemp;           /* running max of W/ext_pp_force.f90:123 */
   FFTPlan *plan_f, *plan_c;
-  /* the coarse PM pipeline runs on `side` beside the fine one on `stream` */
-  hipStream_t side;
-  hipEvent_t ev_fork, ev_join;
+  /* the PM force pipelines run on side streams beside other work: the
+     coarse one beside the fine one in particle_mesh, and both beside PP
+     when ext_pp_force computes them ahead (pm_ahead, for mass pm_mass) */
+  hipStream_t side, side2;
+  hipEvent_t ev_fork, ev_join, ev_join2;
+  int pm_ahead;
+  float pm_mass;
   /* stage timers: event pairs recorded without waiting, resolved when the
      ring is full or the timers are read */
   int profiling;

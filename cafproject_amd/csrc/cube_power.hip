@@ -117,6 +117,7 @@ __global__ void k_pw_zero(double *a, int64_t n) {
 }
 
 extern "C" int cube_power(cube_ctx *c, int32_t ng, int32_t nbin, double *ksum, double *psum, int64_t *count) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
   if (!c || !ksum || !psum || !count || ng < 4 || ng > c->nf || c->nf % ng || nbin < 1) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   if (c->nt > 519) return CUBE_ERR_INVALID; /* k_pw_deposit stages nt+1 row offsets in LDS */
