@@ -108,10 +108,11 @@ MAIN_KERNEL = {
     "fine_fft": "k_r2c_x+k_c2c_r", "fine_inverse": "k_green_r+k_c2c_r+k_c2r_x3",
     "coarse_force": "coarse fft kernels",
 }
-# stages whose kernels share the GPU with another stream's (particle_mesh runs
-# the coarse deposit and force on a side stream beside the fine deposit)
-CONCURRENT = {"fine_deposit": "coarse_deposit+coarse_force", "coarse_deposit": "fine_deposit",
-              "coarse_force": "fine_deposit"}
+# stages whose kernels share the GPU with other streams' (ext_pp_force forms
+# the fine and coarse PM forces on side streams beside the PP kernel)
+CONCURRENT = {"pp": "fine_deposit+fine_fft+fine_inverse+coarse_deposit+coarse_force",
+              "fine_deposit": "pp+coarse_deposit+coarse_force", "coarse_deposit": "pp+fine_deposit+fine_fft",
+              "coarse_force": "pp+fine_fft"}
 # kernel-name prefixes in the rocprofv3 PMC traffic table, per single-kernel stage
 PMC_NAMES = {"pp": ["k_pp_blk<"], "fine_deposit": ["k_fine_deposit_cc<"], "fine_kick": ["k_kick<"],
              "coarse_deposit": ["k_coarse_deposit_b2<"], "coarse_kick": ["k_kick<"]}
