@@ -343,7 +343,10 @@ __global__ void k_place_ranked(const int32_t *__restrict__ dest, const int32_t *
 #define SMALL_SORT 48
 #define BIG_SORT 16384
 #define FB_CELLS 64      /* cells per finalize workgroup (the context sizes fbflag/vpart for it) */
-#define FB_CH 1024       /* slots staged in LDS per finalize pass */
+/* slots staged in LDS per finalize pass: 640 (10 per cell against the 8 of
+   an average cell) keeps the workgroup at ~22 KB of LDS, 7 per CU; a block
+   with more slots takes the chunked path with the global key sort */
+#define FB_CH 640
 #define FB_LDS_SORT 64   /* largest cell sorted inside the finalize workgroup */
 /* flag[b] = 1 when finalize block b (cells b*FB_CELLS...) holds at most FB_CH
    particles and no cell above FB_LDS_SORT: it sorts its cells in LDS */
