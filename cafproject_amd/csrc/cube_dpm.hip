@@ -82,6 +82,9 @@ __device__ static inline float wrapf(float x, float n) {
   x = x - n * floorf(x / n);
   return x >= n ? x - n : x;
 }
+struct V3 { /* three interleaved components, read with one 12-byte load */
+  float x, y, z;
+};
 struct Cic {
   int i[3][2];   /* mesh indices per axis (periodic) */
   float w[3][2]; /* weights */
@@ -143,10 +146,10 @@ __global__ void k_dpm_kick(const float *x, float *p, int64_t np, const float *F,
 #pragma unroll
         for (int xx = 0; xx < 2; ++xx) {
           const float w = c.w[0][xx] * c.w[1][y] * c.w[2][z];
-          const float *m = F + 3 * (((int64_t)c.i[2][z] * n + c.i[1][y]) * n + c.i[0][xx]);
-          f[0] += w * m[0];
-          f[1] += w * m[1];
-          f[2] += w * m[2];
+          const V3 m = *reinterpret_cast<const V3 *>(F + 3 * (((int64_t)c.i[2][z] * n + c.i[1][y]) * n + c.i[0][xx]));
+          f[0] += w * m.x;
+          f[1] += w * m.y;
+          f[2] += w * m.z;
         }
 #pragma unroll
     for (int a = 0; a < 3; ++a) p[3 * q + a] += coef * f[a];
@@ -204,7 +207,10 @@ __global__ void k_dpm_grad(const float *x, int64_t np, int n, const float *lam, 
         for (int xx = 0; xx < 2; ++xx) {
           const int64_t m = ((int64_t)c.i[2][z] * n + c.i[1][y]) * n + c.i[0][xx];
           float v = lam ? s_lam * lam[m] : 0.f;
-          if (M) v += (gv[0] * M[3 * m] + gv[1] * M[3 * m + 1]) + gv[2] * M[3 * m + 2];
+          if (M) {
+            const V3 mv = *reinterpret_cast<const V3 *>(M + 3 * m);
+            v += (gv[0] * mv.x + gv[1] * mv.y) + gv[2] * mv.z;
+          }
           /* dW/dx_a: the weight of axis a replaced by its slope -1 (lower) or +1 (upper) */
           const float sx = xx ? 1.f : -1.f, sy = y ? 1.f : -1.f, sz = z ? 1.f : -1.f;
           acc[0] += v * (sx * c.w[1][y] * c.w[2][z]);

@@ -497,6 +497,9 @@ int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc) {
 /* one workgroup per cell row (tile, k, j): particles of the row are
    contiguous; each thread finds its cell by binary search in LDS */
 #define KT 256
+struct F3 {
+  float x, y, z;
+};
 
 /* a/b for a divisor fixed at compile time, y = RN(1/b): q0 = RN(a*y) and
    Markstein's correction RN(q0 + (a - b*q0)*y) with both steps fused.  For
@@ -601,7 +604,9 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
     for (int q = 0; q < 8; ++q) {
       const int X = cx[q] ? m2[0] : m1[0], Y = cy[q] ? m2[1] : m1[1], Z = cz[q] ? m2[2] : m1[2];
       const float wx = cx[q] ? d2[0] : d1[0], wy = cy[q] ? d2[1] : d1[1], wz = cz[q] ? d2[2] : d1[2];
-      const float *F = force + 3 * (((int64_t)Z * n + Y) * n + X);
+      /* the corner's three components in one 12-byte load */
+      const F3 Fv = *reinterpret_cast<const F3 *>(force + 3 * (((int64_t)Z * n + Y) * n + X));
+      const float F[3] = {Fv.x, Fv.y, Fv.z};
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const float gq = FINE ? div_6pi((F[a] * a_mid) * dt, ok) : F[a];
