@@ -171,6 +171,11 @@ int cube_tile_boxsum(cube_ctx *c, const int32_t *rhoc, int w, int64_t *host_out)
 }
 
 /* ---------------- pass 1: destination cell and order key ---------------- */
+/* three interleaved components, read with one 12-byte load */
+struct V3f {
+  float x, y, z;
+};
+
 struct DriftP {
   Geo g;
   int nlayer;
@@ -227,12 +232,14 @@ __global__ __launch_bounds__(DT) void k_drift_dest(const typename IntOf<IX>::T *
     const int tile[3] = {tx, ty, tz};
     int lcoord[3][3], lig[3][3], loff[3][3], nl[3];
     int err = 0;
+    const V3f vcell = *reinterpret_cast<const V3f *>(vfield + 3 * (c0 + i)); /* one 12-byte load */
+    const float vcf[3] = {vcell.x, vcell.y, vcell.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int32_t xc = (int32_t)xp[3 * p + a];
       const int32_t vc = (int32_t)vp[3 * p + a];
       const double off = ((double)cube_ucode(IX, xc) + 0.5) * P.xres;
-      const double vreal = vdec[vc + P.nvh] + (double)vfield[3 * (c0 + i) + a];
+      const double vreal = vdec[vc + P.nvh] + (double)vcf[a];
       const double dx = ((double)P.dt_mid * vreal) / 4.0;
       nl[a] = 0;
       int cand[3] = {home[a], 0, 0}, coff[3] = {0, 0, 0}, nc = 1;
@@ -570,8 +577,10 @@ __global__ __launch_bounds__(256) void k_drift_final2(
       const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
       sc = cid_global(g, tx * nt + sx, ty * nt + sy, tz * nt + sz);
     }
+    const V3f vo = *reinterpret_cast<const V3f *>(vfield_old + 3 * sc); /* one 12-byte load */
+    const float vof[3] = {vo.x, vo.y, vo.z};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) v[a] = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vfield_old[3 * sc + a];
+    for (int a = 0; a < 3; ++a) v[a] = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vof[a];
   };
 
   float acc[3] = {0.f, 0.f, 0.f};

@@ -718,10 +718,10 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
       f[d] = v;
     }
     if (row0 + l < nrows) {
-      float *o = force + ((row0 + l) * N + x) * 3;
-      o[0] = f[0];
-      o[1] = f[1];
-      o[2] = f[2];
+      struct F3 {
+        float x, y, z;
+      };
+      *reinterpret_cast<F3 *>(force + ((row0 + l) * N + x) * 3) = F3{f[0], f[1], f[2]}; /* one 12-byte store */
       const float f2 = (f[0] * f[0] + f[1] * f[1]) + f[2] * f[2];
       m = fmaxf(m, f2);
     }
