@@ -74,7 +74,11 @@ def test_two_drifts_bit_exact(case, agg_cap, monkeypatch):
     _same_state(g.get_state(), oc.get_state())
 
 
-@pytest.mark.parametrize("case", CASES, ids=_ids)
+# nt = 9 takes the coarse deposit's per-cell kernel, even nt its 2x2x2-block one
+DEP_CASES = CASES + [dict(nc=20, nnt=2), dict(nc=18, nnt=2, izipx=1, izipv=2)]
+
+
+@pytest.mark.parametrize("case", DEP_CASES, ids=_ids)
 def test_fine_and_coarse_deposit_bit_exact(case):
     st, oc, g = _pair(case)
     m = st["mass_p"]
