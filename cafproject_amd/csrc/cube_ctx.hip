@@ -143,7 +143,7 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->spec_f, nspec);
   st |= dalloc(&c->specd_f, 3 * nspec);
   st |= dalloc(&c->force_f, 3 * nf3);
-  st |= dalloc(&c->kern_f, 3 * ho * ho * ho);
+  st |= dalloc(&c->kern_f, cube_kernf_elems(ho));
   const int64_t nc3 = c->ncells, nspec_c = (int64_t)cube_spec_pitch((int)c->nc) * c->nc * c->nc;
   st |= dalloc(&c->rho_c, nc3);
   st |= dalloc(&c->spec_c, nspec_c);
@@ -554,6 +554,32 @@ static int field_ptr(cube_ctx *c, int which, void **ptr, size_t *bytes) {
   }
   return CUBE_ERR_INVALID;
 }
+/* kern_f crosses the boundary as the plain octant [d][kz][ky][kx] of
+   (nf/2+1)^3 per component; on the device it is blocked (cube_kernf_index) */
+static int kernf_transfer(cube_ctx *c, float *out, const float *in) {
+  const int64_t ho = c->nf / 2 + 1, ne = cube_kernf_elems(ho);
+  float *tmp = new (std::nothrow) float[ne];
+  if (!tmp) return CUBE_ERR_ALLOC;
+  int st = CUBE_OK;
+  if (out) {
+    if (hipMemcpy(tmp, c->kern_f, ne * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) st = CUBE_ERR_HIP;
+  } else {
+    memset(tmp, 0, ne * sizeof(float));
+  }
+  for (int d = 0; d < 3 && !st; ++d)
+    for (int kz = 0; kz < ho; ++kz)
+      for (int ky = 0; ky < ho; ++ky) {
+        const int64_t r = ((d * ho + kz) * ho + ky) * ho;
+        for (int kx = 0; kx < ho; ++kx) {
+          const int64_t b = cube_kernf_index(ho, d, kz, ky, kx);
+          if (out) out[r + kx] = tmp[b]; else tmp[b] = in[r + kx];
+        }
+      }
+  if (!st && !out && hipMemcpy(c->kern_f, tmp, ne * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    st = CUBE_ERR_HIP;
+  delete[] tmp;
+  return st;
+}
 extern "C" int cube_get_field(cube_ctx *c, int which, void *host, size_t bytes) {
   if (!c || !host) return CUBE_ERR_INVALID;
   void *p;
@@ -563,6 +589,7 @@ extern "C" int cube_get_field(cube_ctx *c, int which, void *host, size_t bytes) 
   if (bytes != n) return CUBE_ERR_INVALID;
   CUBE_CHECK(hipSetDevice(c->device));
   CUBE_CHECK(hipStreamSynchronize(c->stream));
+  if (which == CUBE_FIELD_KERN_F) return kernf_transfer(c, static_cast<float *>(host), nullptr);
   CUBE_CHECK(hipMemcpy(host, p, n, hipMemcpyDeviceToHost));
   return CUBE_OK;
 }
@@ -576,7 +603,11 @@ extern "C" int cube_set_field(cube_ctx *c, int which, const void *host, size_t b
   if (bytes != n) return CUBE_ERR_INVALID;
   CUBE_CHECK(hipSetDevice(c->device));
   CUBE_CHECK(hipStreamSynchronize(c->stream));
-  CUBE_CHECK(hipMemcpy(p, host, n, hipMemcpyHostToDevice));
+  if (which == CUBE_FIELD_KERN_F) {
+    if ((st = kernf_transfer(c, nullptr, static_cast<const float *>(host)))) return st;
+  } else {
+    CUBE_CHECK(hipMemcpy(p, host, n, hipMemcpyHostToDevice));
+  }
   if (which == CUBE_FIELD_KERN_F || which == CUBE_FIELD_KERN_C) c->has_kernels = 1;
   return CUBE_OK;
 }

@@ -428,10 +428,9 @@ __global__ __launch_bounds__(RC<N>::NT) void k_c2c_r(float2 *__restrict__ a, con
 }
 
 /* Green's function pass along z, register version of k_green_z; OCT selects
-   the octant kernel storage (fine mesh) over the full one (coarse mesh).
-   A workgroup does the lines y and N-y (which read the same kern_f rows of
-   the octant) for two adjacent column blocks (which share kern_f's 128-byte
-   lines), one after the other, so those reads come from its XCD's L2. */
+   the blocked octant kernel storage (fine mesh, cube_kernf_index) over the
+   full one (coarse mesh).  A workgroup does the lines y and N-y, which read
+   the same kernel block, for two adjacent column blocks. */
 template <int N, bool OCT>
 __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict__ spec, float2 *__restrict__ outd,
                                                      const float *__restrict__ kern,
@@ -480,9 +479,10 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
           const float2 vm = m < R2 - KL ? v[m] : keep[(m - (R2 - KL)) * G::NT + tid];
           float K;
           if constexpr (OCT) {
+            constexpr int KB = (int)cube_kernf_blocks(H1);
             const int kz = n <= H ? n : N - n;
-            const float *kb = uni(kern + ((int64_t)d * H1 * H1 + ky) * H1, z0);
-            K = *boff(kb, (uint32_t)((kz * H1 * H1 + kx) * 4));
+            const float *kb = uni(kern + ((int64_t)d * H1 + ky) * KB * H1 * 16, z0);
+            K = *boff(kb, (uint32_t)((((kx >> 4) * H1 + kz) * 16 + (kx & 15)) * 4));
             if (d == 1) K *= sy;
             if (d == 2 && n > H) K = -K;
           } else {
@@ -577,8 +577,9 @@ __global__ __launch_bounds__(FFT_T) void k_c2c_s(float2 *__restrict__ a, const f
 
 /* Green's function pass along z.  spec: [z][y][x'] (x' < NH).  For each
    (x',y) line: forward FFT along z, then for d: out_d = IFFT_z(i K_d * .).
-   kern: octant storage [d][kz'][ky'][kx] of (N/2+1)^3 (octant=1, odd in the
-   d-th wave number) or full storage [d][kz][ky][kx] of N*N*NH (octant=0). */
+   kern: blocked octant storage of (N/2+1)^3 per component (octant=1, odd in
+   the d-th wave number, cube_kernf_index) or full storage [d][kz][ky][kx] of
+   N*N*NH (octant=0). */
 template <int N>
 __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ spec, float2 *__restrict__ outd,
                                                  const float *__restrict__ kern, int octant,
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ sp
           if (octant) {
             const int kz = n <= H ? n : N - n;
             const float sz = n <= H ? 1.f : -1.f;
-            K = kern[(((int64_t)d * (H + 1) + kz) * (H + 1) + ky) * (H + 1) + kx];
+            K = kern[cube_kernf_index(H + 1, d, kz, ky, kx)];
             if (d == 1) K *= sy;
             if (d == 2) K *= sz;
           } else {

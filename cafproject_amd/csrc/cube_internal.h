@@ -75,7 +75,7 @@ struct cube_ctx {
   float2 *spec_f;           /* pitch*nf*nf */
   float2 *specd_f;          /* 3 * pitch*nf*nf */
   float *force_f;           /* 3*nf^3, (d fastest) */
-  float *kern_f;            /* 3*(nf/2+1)^3 octant */
+  float *kern_f;            /* octant in 16-column blocks, cube_kernf_index */
   /* coarse mesh */
   float *rho_c;
   float2 *spec_c, *specd_c;
@@ -216,6 +216,18 @@ __device__ __forceinline__ int32_t cube_encode_v_tab(double c, double v, const d
    values padded to a multiple of 16 (128 bytes), so every row and every
    16-column batch of the strided FFT passes starts on a cache line. */
 __host__ __device__ constexpr int cube_spec_pitch(int n) { return ((n / 2 + 1) + 15) / 16 * 16; }
+/* kern_f storage.  The fine Green's function is kept for the octant
+   0 <= kx, ky, kz <= nf/2 (odd in k_d, the other octants follow by sign) as,
+   for each component d and ky, blocks of 16 kx columns that are each one
+   contiguous [kz][16] run: element (d, kz, ky, kx) sits at
+   (((d*ho + ky)*KB + kx/16)*ho + kz)*16 + kx%16, ho = nf/2+1, KB = ceil(ho/16).
+   The z pass over 16 spectrum columns then streams its kernel values instead
+   of reading 64-byte pieces of unaligned 513-float rows. */
+__host__ __device__ constexpr int64_t cube_kernf_blocks(int64_t ho) { return (ho + 15) / 16; }
+__host__ __device__ constexpr int64_t cube_kernf_elems(int64_t ho) { return 3 * ho * cube_kernf_blocks(ho) * ho * 16; }
+__host__ __device__ constexpr int64_t cube_kernf_index(int64_t ho, int d, int kz, int ky, int kx) {
+  return (((d * ho + ky) * cube_kernf_blocks(ho) + (kx >> 4)) * ho + kz) * 16 + (kx & 15);
+}
 
 /* launchers (one file each) */
 int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi, cube_vstats *out);

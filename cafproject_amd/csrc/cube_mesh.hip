@@ -731,13 +731,16 @@ __global__ void k_fine_kernel_rs(float *rs, int n, const float *tab, int d, int 
     rs[q] = v;
   }
 }
-/* keep Im of the positive-frequency octant: kern[d][kz][ky][kx], 0..n/2 */
+/* keep Im of the positive-frequency octant 0..n/2 of component d in the
+   blocked storage of cube_kernf_index (columns past n/2 of the last block
+   hold zero) */
 __global__ void k_extract_octant(const float2 *spec, int n, float *kern, int d) {
   const int h = n / 2 + 1;
-  const int64_t tot = (int64_t)h * h * h;
+  const int64_t kb = cube_kernf_blocks(h), tot = (int64_t)h * kb * h * 16;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
-    const int kx = (int)(q % h), ky = (int)((q / h) % h), kz = (int)(q / ((int64_t)h * h));
-    kern[(int64_t)d * tot + q] = spec[((int64_t)kz * n + ky) * cube_spec_pitch(n) + kx].y;
+    const int kz = (int)((q / 16) % h), b = (int)((q / (16 * h)) % kb), ky = (int)(q / (16 * h * kb));
+    const int kx = b * 16 + (int)(q % 16);
+    kern[(int64_t)d * tot + q] = kx < h ? spec[((int64_t)kz * n + ky) * cube_spec_pitch(n) + kx].y : 0.f;
   }
 }
 /* uncorrected coarse kernel -r/r^3 in fine units (W/kernel_c.f90:16-28) and
