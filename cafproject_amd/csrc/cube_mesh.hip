@@ -241,10 +241,12 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
             }
             /* a source cell below C on an axis reaches C's 4^3 fine cells
                only through particles in its last fine sub-cell on that axis
-               (code >> sh == 3; the rounding of tempx near the cell's end
-               keeps that code), so those visits first mark the candidates
+               (code >> sh == 3), so those visits first mark the candidates
                of each 64-particle chunk in a mask and then visit only them,
-               in increasing l */
+               in increasing l.  Where the real(4) tempx is inexact (2-byte
+               codes, nt > 128) the top code of a lower sub-cell can round up
+               onto the boundary itself, and its weight past the boundary is
+               then exactly 0: skipping it drops only +0 terms. */
             const int sh = IX == 1 ? 6 : 14;
             const bool ex = qx == 0, ey = qy == 0, ez = ci[2] == C[2] - 1;
             for (int lb = 0; lb < np; lb += 64) {
