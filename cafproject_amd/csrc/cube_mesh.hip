@@ -583,17 +583,17 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
       d2[a] = 1.0f - d1[a];
       vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
     }
-    /* 0-based periodic mesh coordinates of the two CIC planes per axis */
+    /* 0-based periodic mesh coordinates of the two CIC planes per axis.
+       tempx lies in (4(loc-1), 4 loc] (fine) or (g-1/2, g+1/2] around the
+       global cell g (coarse), so the first plane is in [0, nfine] or
+       [-1, nc-1] and one compare wraps each plane (no integer modulo) */
     int m1[3], m2[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (FINE) {
-        m1[a] = wrapi(tt[a] * nft + i1[a] - 1, nfine);
-        m2[a] = wrapi(tt[a] * nft + i1[a], nfine);
-      } else {
-        m1[a] = wrapi(i1[a] - 1, g.nc);
-        m2[a] = wrapi(i1[a], g.nc);
-      }
+      const int n = FINE ? nfine : g.nc;
+      const int v = FINE ? tt[a] * nft + i1[a] - 1 : i1[a] - 1;
+      m1[a] = v < 0 ? v + n : (v >= n ? v - n : v);
+      m2[a] = v + 1 >= n ? v + 1 - n : v + 1;
     }
     const int n = FINE ? nfine : g.nc;
     /* corner order of W/pm.f90:146-153 */
