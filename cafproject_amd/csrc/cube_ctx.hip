@@ -508,13 +508,13 @@ extern "C" int cube_particle_mesh(cube_ctx *c, float a_mid, float dt, float sigm
   c->pm_ahead = 0;
   if (!ahead && (st = pm_forces(c, mass_p, c->stream, c->side))) return st;
   CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
+  /* the fine kick, sigma_vi = sigma_vi_new (W/pm.f90:193), the coarse kick:
+     one pass over the particles, timed as the fine kick */
   tbeg(c);
-  st = cube_fine_kick(c, a_mid, dt, sigma_vi, sigma_vi_new, 0);
+  st = cube_pm_kicks(c, a_mid, dt, sigma_vi, sigma_vi_new, o.vmax); /* synchronises */
   tend(c, CUBE_T_FINE_KICK);
   if (st) return st;
-  sigma_vi = sigma_vi_new; /* W/pm.f90:193 */
-  CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
-  if ((st = cube_pm_coarse_kick(c, a_mid, dt, sigma_vi, o.vmax))) return st; /* synchronises */
   {
     double sums[2];
     unsigned f2[2];

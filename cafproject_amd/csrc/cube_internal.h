@@ -251,6 +251,7 @@ int cube_fine_deposit(cube_ctx *c, float mass_p, double *testrhof);
 int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new, int sync);
 int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc);
 int cube_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax);
+int cube_pm_kicks(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new, float *vmax);
 int cube_green_init(cube_ctx *c, const float *wfxyzf, const float *wfxyzc);
 int cube_pp(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, float *f2_max_pp,
             int64_t *itest1);

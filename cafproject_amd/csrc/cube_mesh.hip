@@ -496,8 +496,6 @@ int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc) {
 }
 
 /* ---------------- kicks ---------------- */
-/* one workgroup per cell row (tile, k, j): particles of the row are
-   contiguous; each thread finds its cell by binary search in LDS */
 #define KT 256
 struct F3 {
   float x, y, z;
@@ -542,13 +540,107 @@ __global__ void k_kick_prescale(const float *__restrict__ F, int64_t n, float a_
   }
 }
 
-template <int IX, int IV, bool FINE>
+/* One kick of one particle: vreal[a] += the CIC gather of the mesh over
+   the 8 corners in the order of W/pm.f90:146-153 (fine: force_f with
+   ((((F*a_mid)*dt)/6)/pi)*wx*wy*wz per term; coarse: the prescaled G with
+   G*wx*wy*wz, W/pm.f90:341-348).  loc: 1-based cell in the tile, tt: tile. */
+template <int IX, bool FINE>
+__device__ __forceinline__ void kick_particle(const int32_t code[3], const int loc[3], const int tt[3],
+                                              const Geo &g, const float *__restrict__ force, float a_mid,
+                                              float dt, double vreal[3]) {
+  const int nt = g.nt, nfine = 4 * g.nc, nft = 4 * nt;
+  int i1[3];
+  float d1[3], d2[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float tempx;
+    if (FINE) /* ncell*((/i,j,k/)-1)+ncell*(...)*x_resolution, tile frame (W/pm.f90:138) */
+      tempx = (float)((double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, code[a]));
+    else /* ((/itx,../)-1)*nt+((/i,../)-1)+(...)*x_resolution-0.5 (W/pm.f90:335) */
+      tempx = (float)(((double)(tt[a] * nt + (loc[a] - 1)) + cube_xoff(IX, code[a])) - 0.5);
+    i1[a] = (int)floorf(tempx) + 1;
+    d1[a] = (float)i1[a] - tempx;
+    d2[a] = 1.0f - d1[a];
+  }
+  /* 0-based periodic mesh coordinates of the two CIC planes per axis.
+     tempx lies in (4(loc-1), 4 loc] (fine) or (g-1/2, g+1/2] around the
+     global cell g (coarse), so the first plane is in [0, nfine] or
+     [-1, nc-1] and one compare wraps each plane (no integer modulo) */
+  const int n = FINE ? nfine : g.nc;
+  int m1[3], m2[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int v = FINE ? tt[a] * nft + i1[a] - 1 : i1[a] - 1;
+    m1[a] = v < 0 ? v + n : (v >= n ? v - n : v);
+    m2[a] = v + 1 >= n ? v + 1 - n : v + 1;
+  }
+  /* corner order of W/pm.f90:146-153 */
+  const int cx[8] = {0, 1, 0, 0, 0, 1, 1, 1}, cy[8] = {0, 0, 1, 0, 1, 0, 1, 1}, cz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
+  const double v0[3] = {vreal[0], vreal[1], vreal[2]};
+  bool ok = true;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int X = cx[q] ? m2[0] : m1[0], Y = cy[q] ? m2[1] : m1[1], Z = cz[q] ? m2[2] : m1[2];
+    const float wx = cx[q] ? d2[0] : d1[0], wy = cy[q] ? d2[1] : d1[1], wz = cz[q] ? d2[2] : d1[2];
+    /* the corner's three components in one 12-byte load */
+    const F3 Fv = *reinterpret_cast<const F3 *>(force + 3 * (((int64_t)Z * n + Y) * n + X));
+    const float F[3] = {Fv.x, Fv.y, Fv.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float gq = FINE ? div_6pi((F[a] * a_mid) * dt, ok) : F[a];
+      const float inc = ((gq * wx) * wy) * wz;
+      vreal[a] = vreal[a] + (double)inc;
+    }
+  }
+  if (FINE && !ok) { /* a quotient outside the checked range: redo with divisions */
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vreal[a] = v0[a];
+    for (int q = 0; q < 8; ++q) {
+      const int X = cx[q] ? m2[0] : m1[0], Y = cy[q] ? m2[1] : m1[1], Z = cz[q] ? m2[2] : m1[2];
+      const float wx = cx[q] ? d2[0] : d1[0], wy = cy[q] ? d2[1] : d1[1], wz = cz[q] ? d2[2] : d1[2];
+      const float *F = force + 3 * (((int64_t)Z * n + Y) * n + X);
+      for (int a = 0; a < 3; ++a) {
+        const float inc = ((((((F[a] * a_mid) * dt) / 6.0f) / CUBE_PI_F) * wx) * wy) * wz;
+        vreal[a] = vreal[a] + (double)inc;
+      }
+    }
+  }
+}
+
+/* max of |v + v_c| over the workgroup: wave shuffles, then the waves
+   through LDS, one atomic per component (W/pm.f90:349) */
+__device__ __forceinline__ void vmax_reduce(double vm[3], unsigned long long *__restrict__ vmax_bits) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vm[a] = fmax(vm[a], __shfl_xor(vm[a], o, 64));
+  __shared__ double red[3][KT / 64];
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) red[a][threadIdx.x >> 6] = vm[a];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double m = red[threadIdx.x][0];
+    for (int w = 1; w < KT / 64; ++w) m = fmax(m, red[threadIdx.x][w]);
+    atomicMax(vmax_bits + threadIdx.x, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+/* kicks, one workgroup per cell row (tile, k, j): particles of the row are
+   contiguous; each thread finds its cell by binary search in LDS.
+   NKICK = 1: the one kick of FINE (decode with vdec, encode with venc);
+   NKICK = 2: the fine kick (vdec, venc), then the coarse one on the
+   re-decoded code (vdec2, venc2) in the same pass, which is the step's
+   sequence W/pm.f90:129-159 then 322-354 with the intermediate codes kept
+   in registers instead of going through vp. */
+template <int IX, int IV, bool FINE, int NKICK>
 __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__restrict__ xp,
                                              typename IntOf<IV>::T *__restrict__ vp,
                                              const int64_t *__restrict__ cstart, const float *__restrict__ vfield,
                                              const float *__restrict__ force, const double *__restrict__ vdec,
                                              Geo g, int nvh, float a_mid, float dt, double venc,
-                                             unsigned long long *__restrict__ vmax_bits) {
+                                             unsigned long long *__restrict__ vmax_bits,
+                                             const float *__restrict__ force2, const double *__restrict__ vdec2,
+                                             double venc2) {
   const int nt = g.nt;
   const int64_t row = blockIdx.x;
   const int j = (int)(row % nt), k = (int)((row / nt) % nt), t = (int)(row / ((int64_t)nt * nt));
@@ -557,7 +649,7 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
   const int64_t c0 = row * nt;
   for (int i = threadIdx.x; i <= nt; i += KT) cs[i] = cstart[c0 + i];
   __syncthreads();
-  const int nfine = 4 * g.nc, nft = 4 * nt;
+  constexpr bool VMAX = !FINE || NKICK == 2; /* the coarse kick reports vmax */
   double vm[3] = {0, 0, 0};
   for (int64_t p = cs[0] + threadIdx.x; p < cs[nt]; p += KT) {
     int lo = 0, hi = nt;
@@ -567,111 +659,54 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
     }
     const int loc[3] = {lo + 1, j + 1, k + 1};
     const int tt[3] = {tx, ty, tz};
-    int i1[3];
-    float d1[3], d2[3];
+    int32_t code[3];
     double vreal[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const int32_t code = (int32_t)xp[3 * p + a];
-      float tempx;
-      if (FINE) /* ncell*((/i,j,k/)-1)+ncell*(...)*x_resolution, tile frame (W/pm.f90:138) */
-        tempx = (float)((double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, code));
-      else /* ((/itx,../)-1)*nt+((/i,../)-1)+(...)*x_resolution-0.5 (W/pm.f90:335) */
-        tempx = (float)(((double)(tt[a] * nt + (loc[a] - 1)) + cube_xoff(IX, code)) - 0.5);
-      i1[a] = (int)floorf(tempx) + 1;
-      d1[a] = (float)i1[a] - tempx;
-      d2[a] = 1.0f - d1[a];
+      code[a] = (int32_t)xp[3 * p + a];
       vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
     }
-    /* 0-based periodic mesh coordinates of the two CIC planes per axis.
-       tempx lies in (4(loc-1), 4 loc] (fine) or (g-1/2, g+1/2] around the
-       global cell g (coarse), so the first plane is in [0, nfine] or
-       [-1, nc-1] and one compare wraps each plane (no integer modulo) */
-    int m1[3], m2[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const int n = FINE ? nfine : g.nc;
-      const int v = FINE ? tt[a] * nft + i1[a] - 1 : i1[a] - 1;
-      m1[a] = v < 0 ? v + n : (v >= n ? v - n : v);
-      m2[a] = v + 1 >= n ? v + 1 - n : v + 1;
-    }
-    const int n = FINE ? nfine : g.nc;
-    /* corner order of W/pm.f90:146-153 */
-    const int cx[8] = {0, 1, 0, 0, 0, 1, 1, 1}, cy[8] = {0, 0, 1, 0, 1, 0, 1, 1}, cz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
-    /* vreal += ((((((F*a_mid)*dt)/6)/pi)*wx)*wy)*wz per corner, W/pm.f90:146-153;
-       the coarse mesh arrives as G = (((F*a_mid)*dt)/6)/pi */
-    const double v0[3] = {vreal[0], vreal[1], vreal[2]};
-    bool ok = true;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int X = cx[q] ? m2[0] : m1[0], Y = cy[q] ? m2[1] : m1[1], Z = cz[q] ? m2[2] : m1[2];
-      const float wx = cx[q] ? d2[0] : d1[0], wy = cy[q] ? d2[1] : d1[1], wz = cz[q] ? d2[2] : d1[2];
-      /* the corner's three components in one 12-byte load */
-      const F3 Fv = *reinterpret_cast<const F3 *>(force + 3 * (((int64_t)Z * n + Y) * n + X));
-      const float F[3] = {Fv.x, Fv.y, Fv.z};
+    kick_particle<IX, FINE>(code, loc, tt, g, force, a_mid, dt, vreal);
+    if (NKICK == 2) {
+      /* the code the fine kick would store, decoded again for the coarse kick */
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float gq = FINE ? div_6pi((F[a] * a_mid) * dt, ok) : F[a];
-        const float inc = ((gq * wx) * wy) * wz;
-        vreal[a] = vreal[a] + (double)inc;
+        const int32_t v1 = (int32_t)(typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal[a]);
+        vreal[a] = vdec2[v1 + nvh];
       }
+      kick_particle<IX, false>(code, loc, tt, g, force2, a_mid, dt, vreal);
     }
-    if (FINE && !ok) { /* a quotient outside the checked range: redo with divisions */
-#pragma unroll
-      for (int a = 0; a < 3; ++a) vreal[a] = v0[a];
-      for (int q = 0; q < 8; ++q) {
-        const int X = cx[q] ? m2[0] : m1[0], Y = cy[q] ? m2[1] : m1[1], Z = cz[q] ? m2[2] : m1[2];
-        const float wx = cx[q] ? d2[0] : d1[0], wy = cy[q] ? d2[1] : d1[1], wz = cz[q] ? d2[2] : d1[2];
-        const float *F = force + 3 * (((int64_t)Z * n + Y) * n + X);
-        for (int a = 0; a < 3; ++a) {
-          const float inc = ((((((F[a] * a_mid) * dt) / 6.0f) / CUBE_PI_F) * wx) * wy) * wz;
-          vreal[a] = vreal[a] + (double)inc;
-        }
-      }
-    }
+    const double ve = NKICK == 2 ? venc2 : venc;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (!FINE) {
+      if (VMAX) {
         const double av = fabs(vreal[a] + (double)vfield[3 * (c0 + lo) + a]);
         vm[a] = av > vm[a] ? av : vm[a];
       }
-      vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal[a]);
+      vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v(IV, ve, vreal[a]);
     }
   }
-  if (!FINE) { /* max over the workgroup: wave shuffles, then the waves through LDS */
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) vm[a] = fmax(vm[a], __shfl_xor(vm[a], o, 64));
-    __shared__ double red[3][KT / 64];
-    if ((threadIdx.x & 63) == 0)
-      for (int a = 0; a < 3; ++a) red[a][threadIdx.x >> 6] = vm[a];
-    __syncthreads();
-    if (threadIdx.x < 3) {
-      double m = red[threadIdx.x][0];
-      for (int w = 1; w < KT / 64; ++w) m = fmax(m, red[threadIdx.x][w]);
-      atomicMax(vmax_bits + threadIdx.x, (unsigned long long)__double_as_longlong(m));
-    }
-  }
+  if (VMAX) vmax_reduce(vm, vmax_bits);
 }
 
-template <int IX, int IV, bool FINE>
+template <int IX, int IV, bool FINE, int NKICK>
 static int kick_impl(cube_ctx *c, const float *force, float a_mid, float dt, double venc,
-                     unsigned long long *vmb) {
+                     unsigned long long *vmb, const float *force2, double venc2) {
   const int64_t nrows = c->ncells / c->nt;
-  hipLaunchKernelGGL((k_kick<IX, IV, FINE>), dim3((unsigned)nrows), dim3(KT), 0, c->stream,
+  hipLaunchKernelGGL((k_kick<IX, IV, FINE, NKICK>), dim3((unsigned)nrows), dim3(KT), 0, c->stream,
                      (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->cstart, c->vfield, force,
-                     c->vdec0, geo_of(c), c->nvbin / 2, a_mid, dt, venc, vmb);
+                     c->vdec0, geo_of(c), c->nvbin / 2, a_mid, dt, venc, vmb, force2, c->vdec1, venc2);
   CUBE_LAUNCH_CHECK();
   return CUBE_OK;
 }
-template <bool FINE>
-static int kick(cube_ctx *c, const float *force, float a_mid, float dt, double venc, unsigned long long *vmb) {
+template <bool FINE, int NKICK = 1>
+static int kick(cube_ctx *c, const float *force, float a_mid, float dt, double venc, unsigned long long *vmb,
+                const float *force2 = nullptr, double venc2 = 0.0) {
   const int x = c->p.izipx, v = c->p.izipv;
-  if (x == 1 && v == 1) return kick_impl<1, 1, FINE>(c, force, a_mid, dt, venc, vmb);
-  if (x == 1) return kick_impl<1, 2, FINE>(c, force, a_mid, dt, venc, vmb);
-  if (v == 1) return kick_impl<2, 1, FINE>(c, force, a_mid, dt, venc, vmb);
-  return kick_impl<2, 2, FINE>(c, force, a_mid, dt, venc, vmb);
+  if (x == 1 && v == 1) return kick_impl<1, 1, FINE, NKICK>(c, force, a_mid, dt, venc, vmb, force2, venc2);
+  if (x == 1) return kick_impl<1, 2, FINE, NKICK>(c, force, a_mid, dt, venc, vmb, force2, venc2);
+  if (v == 1) return kick_impl<2, 1, FINE, NKICK>(c, force, a_mid, dt, venc, vmb, force2, venc2);
+  return kick_impl<2, 2, FINE, NKICK>(c, force, a_mid, dt, venc, vmb, force2, venc2);
 }
 
 /* fine kick, W/pm.f90:129-159: decode with sigma_vi, encode with sigma_vi_new */
@@ -686,6 +721,35 @@ int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sig
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+/* The step's two kicks in one pass over the particles (W/pm.f90:129-159,
+   then :193 and :322-387): the fine kick decodes with sigma_vi and encodes
+   with sigma_vi_new, the coarse kick decodes and encodes with sigma_vi_new;
+   vmax (W/pm.f90:349) as cube_coarse_kick.  Equal to cube_fine_kick then
+   cube_coarse_kick(sigma_vi_new) bit for bit. */
+int cube_pm_kicks(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new, float *vmax) {
+  int st = cube_vdec_table(c, c->vdec0, sigma_vi);
+  if (!st) st = cube_vdec_table(c, c->vdec1, sigma_vi_new);
+  if (st) return st;
+  unsigned long long *vmb = (unsigned long long *)c->ured + 8;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, vmb, 3);
+  CUBE_LAUNCH_CHECK();
+  float *G = reinterpret_cast<float *>(c->specd_c);
+  hipLaunchKernelGGL(k_kick_prescale, dim3(2048), dim3(256), 0, c->stream, c->force_c, 3 * c->ncells, a_mid, dt, G);
+  CUBE_LAUNCH_CHECK();
+  const double venc = cube_vconst(sigma_vi_new);
+  st = kick<true, 2>(c, c->force_f, a_mid, dt, venc, vmb, G, venc);
+  if (st) return st;
+  unsigned long long h[3];
+  CUBE_CHECK(hipMemcpyAsync(h, vmb, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  CUBE_CHECK(hipStreamSynchronize(c->stream));
+  for (int a = 0; a < 3; ++a) {
+    double d;
+    memcpy(&d, &h[a], sizeof(d));
+    if (vmax) vmax[a] = (float)d; /* vmax is real(4) (W/variables.f90:35) */
+  }
+  return CUBE_OK;
 }
 
 /* coarse kick, W/pm.f90:322-387: decode and encode with sigma_vi */

@@ -190,10 +190,11 @@ enum {
   CUBE_T_FINE_GREEN = 5,   /* fine z pass: fft, i*K_d, inverse fft (x3)   */
   CUBE_T_FINE_INVY = 6,    /* fine inverse y passes (x3)                   */
   CUBE_T_FINE_C2R = 7,     /* fine c2r x of the three components + max F^2 */
-  CUBE_T_FINE_KICK = 8,    /* fine force gather + kick                     */
+  CUBE_T_FINE_KICK = 8,    /* fine force gather + kick; in particle_mesh
+                              both kicks, which run as one pass            */
   CUBE_T_COARSE_DEPOSIT = 9,
   CUBE_T_COARSE_FORCE = 10,
-  CUBE_T_COARSE_KICK = 11,
+  CUBE_T_COARSE_KICK = 11, /* the coarse kick called on its own            */
   CUBE_T_CHECKS = 12,      /* overflow checks (tile/image/pp box sums)     */
   CUBE_NTIMERS = 13
 };
