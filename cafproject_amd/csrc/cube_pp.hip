@@ -200,23 +200,37 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     p0 = cstart[cq];
   }
   __syncthreads();
-  /* the region cell of particle l of this thread's staged cell, or -1 */
-  auto region = [&](int l, int32_t code[3]) -> int {
+  /* the region cell of a particle of this thread's staged cell, or -1 */
+  auto region = [&](const int32_t code[3]) -> int {
     bool in = true;
     int r[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      code[a] = (int32_t)xp[3 * (p0 + l) + a];
       r[a] = 4 * (sc[a] - 1) + (int)(cube_ucode(IX, code[a]) >> sh) + 1 - flo[a];
       in = in && r[a] >= 0 && r[a] < RF;
     }
     return in ? (r[2] * RF + r[1]) * RF + r[0] : -1;
   };
+  /* the codes of particles l0, l0+d, l0+2d, l0+3d (clamped to [0, np)) in
+     flight together */
+  auto codes4 = [&](int l0, int d, int32_t c4[4][3]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int l = min(max(l0 + d * u, 0), np - 1);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) c4[u][a] = (int32_t)xp[3 * (p0 + l) + a];
+    }
+  };
   /* count per fine cell: the cell's owner thread only, no atomics */
-  for (int l = 0; l < np; ++l) {
-    int32_t code[3];
-    const int ri = region(l, code);
-    if (ri >= 0) cur[ri] = (uint16_t)(cur[ri] + 1);
+  for (int l0 = 0; l0 < np; l0 += 4) {
+    int32_t c4[4][3];
+    codes4(l0, 1, c4);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (l0 + u >= np) break;
+      const int ri = region(c4[u]);
+      if (ri >= 0) cur[ri] = (uint16_t)(cur[ri] + 1);
+    }
   }
   __syncthreads();
   /* exclusive scan of the counts */
@@ -237,16 +251,21 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   if (staged) {
     /* chain order inside a fine cell is decreasing l: walk l downwards and
        append; afterwards cur[f] is the end of fine cell f's list */
-    for (int l = np - 1; l >= 0; --l) {
-      int32_t code[3];
-      const int ri = region(l, code);
-      if (ri < 0) continue;
-      const int slot = cur[ri];
-      cur[ri] = (uint16_t)(slot + 1);
-      int r[3];
+    for (int l0 = np - 1; l0 >= 0; l0 -= 4) {
+      int32_t c4[4][3];
+      codes4(l0, -1, c4);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) r[a] = pp_fix(IX, sc[a], code[a]) - X0[a];
-      pos[slot] = pp_pack(r);
+      for (int u = 0; u < 4; ++u) {
+        if (l0 - u < 0) break;
+        const int ri = region(c4[u]);
+        if (ri < 0) continue;
+        const int slot = cur[ri];
+        cur[ri] = (uint16_t)(slot + 1);
+        int r[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[a] = pp_fix(IX, sc[a], c4[u][a]) - X0[a];
+        pos[slot] = pp_pack(r);
+      }
     }
   }
   if (tid == 0) {
@@ -264,8 +283,9 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   unsigned long long ntest = 0;
   const int ntot = rowoff[16];
   for (int i = tid; i < ntot; i += PPB_T) {
-    int row = 0;
-    while (rowoff[row + 1] <= i) ++row;
+    int row = 0; /* the last of the 16 rows with rowoff[row] <= i */
+#pragma unroll
+    for (int st = 8; st > 0; st >>= 1) row += rowoff[row + st] <= i ? st : 0;
     const int64_t p = cst[row][0] + (i - rowoff[row]);
     int cx = 0;
     while (cx < 3 && cst[row][cx + 1] <= p) ++cx;
@@ -289,16 +309,23 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
          pair count of the wave, not 27 times the largest per-cell count. */
       const int RW = 2 * pr + 1;
       const int f00 = ((gf[2] - pr - flo[2]) * RF + (gf[1] - pr - flo[1])) * RF + (gf[0] - pr - flo[0]);
+      /* row (kk, jj) starts at f00 + (kk*RF + jj)*RF: stepped by counters,
+         not by dividing by the run-time RW */
       int n = 0;
-      for (int r = 0; r < RW * RW; ++r) {
-        const int f0 = f00 + ((r / RW) * RF + r % RW) * RF;
-        n += (int)cur[f0 + RW - 1] - (int)cur[f0 - 1];
-      }
-      int r = -1, base = 0, cend = 0; /* pair t < cend lives at pos slot base + t */
+      for (int kk = 0; kk < RW; ++kk)
+        for (int jj = 0; jj < RW; ++jj) {
+          const int f0 = f00 + (kk * RF + jj) * RF;
+          n += (int)cur[f0 + RW - 1] - (int)cur[f0 - 1];
+        }
+      int f0 = f00 - RF, jr = -1, base = 0, cend = 0; /* pair t < cend lives at pos slot base + t */
       for (int t = 0; t < n; ++t) {
         while (t >= cend) {
-          ++r;
-          const int f0 = f00 + ((r / RW) * RF + r % RW) * RF;
+          if (++jr == RW) { /* next kk */
+            jr = 0;
+            f0 += (RF - RW + 1) * RF;
+          } else {
+            f0 += RF;
+          }
           const int s0 = cur[f0 - 1], e = cur[f0 + RW - 1];
           base = s0 - cend;
           cend += e - s0;

@@ -168,6 +168,28 @@ def test_pp_bit_exact(case, stage_max, monkeypatch):
     assert f_g == f_o
 
 
+@pytest.mark.parametrize("pp_range", [0, 2])
+@pytest.mark.parametrize("stage_max", [None, "0"])
+def test_pp_other_ranges_bit_exact(pp_range, stage_max, monkeypatch):
+    """pp_range 0 and 2: 1 and 25 (kk, jj) rows of 1 and 5 fine cells per
+    particle, staged and global walks."""
+    from cafproject_amd.cube import Cube
+    if stage_max is not None:
+        monkeypatch.setenv("CUBE_PP_STAGE_MAX", stage_max)
+    nc, nnt = 24, 2
+    st = zeldovich_state(nc, nnt, seed=43)
+    oc = O.OracleCube(nc, nnt, pp_on=True, pp_range=pp_range)
+    oc.set_state(st["xp"], st["vp"], st["pid"], st["rhoc"], st["vfield"])
+    oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+    g = Cube(nc, nnt, pp_on=True, pp_range=pp_range)
+    g.set_state(st["xp"], st["vp"], st["pid"], st["rhoc"], st["vfield"])
+    m = st["mass_p"]
+    f_o = oc.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
+    f_g = g.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
+    _same_state(g.get_state(), oc.get_state())
+    assert f_g == f_o
+
+
 def test_coarse_force_small_box_is_finite():
     """nc < 24 is outside the reference's range (W/parameters.f90:29): its
     LRCKCORR divides by the zero Nyquist mode.  The library keeps those
