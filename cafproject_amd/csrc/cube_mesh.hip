@@ -182,6 +182,20 @@ __global__ void k_sum_parts1(const double *part, int nb, double *out) {
  * bit-identical; each particle is read by the 8 threads whose blocks its
  * cloud can touch instead of by all 64+ fine cells around it.
  */
+/* global coordinates of cell cg of the nc^3 x-fastest mesh (32-bit
+   divisions while the index fits, which is every mesh up to 1290^3) */
+__device__ __forceinline__ void gcoords(int64_t cg, int nc, int gg[3]) {
+  if (cg <= 0x7fffffff) {
+    const uint32_t c = (uint32_t)cg, n = (uint32_t)nc, q = c / n, z = q / n;
+    gg[0] = (int)(c - q * n);
+    gg[1] = (int)(q - z * n);
+    gg[2] = (int)z;
+  } else {
+    gg[0] = (int)(cg % nc);
+    gg[1] = (int)((cg / nc) % nc);
+    gg[2] = (int)(cg / ((int64_t)nc * nc));
+  }
+}
 #define FD_T 128
 template <int IX>
 __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<IX>::T *__restrict__ xp,
@@ -197,12 +211,26 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
     const bool live = cg < nc3;
     for (int q = 0; q < 64; ++q) acc[q * FD_T + t] = 0.f;
     if (live) {
-      const int gg[3] = {(int)(cg % nc), (int)((cg / nc) % nc), (int)(cg / ((int64_t)nc * nc))};
+      int gg[3];
+      gcoords(cg, nc, gg);
       int T[3], C[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         T[a] = gg[a] / g.nt;
         C[a] = gg[a] - T[a] * g.nt + 1; /* 1-based local coarse cell */
+      }
+      /* cell ids are linear in the local coordinates and the tile index, so
+         a source below C on an axis is C's id plus that axis's step down:
+         -stride inside the tile, the wrapped neighbour tile's last cell at
+         the tile's low edge */
+      const int64_t nt = g.nt, nt3 = nt * nt * nt;
+      const int64_t cidC = g.cid(C[0] - 1, C[1] - 1, C[2] - 1, T[0] + g.nnt * (T[1] + g.nnt * T[2]));
+      int64_t dm[3];
+      {
+        const int64_t stride[3] = {1, nt, nt * nt}, tstride[3] = {nt3, nt3 * g.nnt, nt3 * g.nnt * g.nnt};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          dm[a] = C[a] > 1 ? -stride[a] : stride[a] * (nt - 1) + tstride[a] * (T[a] > 0 ? -1 : g.nnt - 1);
       }
       /* the two source z cells in (ilayer, k) order, ilayer = (k-k0) mod 3 */
       int za = C[2] - 1, zb = C[2];
@@ -219,8 +247,8 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
       int64_t p0c[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int64_t cc = cid_global(g, T[0] * g.nt + C[0] - 2 + (q & 1), T[1] * g.nt + C[1] - 2 + ((q >> 1) & 1),
-                                      T[2] * g.nt + ((q >> 2) ? zb : za) - 1);
+        const int64_t cc = cidC + ((q & 1) ? 0 : dm[0]) + (((q >> 1) & 1) ? 0 : dm[1]) +
+                           ((((q >> 2) ? zb : za) == C[2]) ? 0 : dm[2]);
         npc[q] = rhoc[cc];
         p0c[q] = cstart[cc];
       }
