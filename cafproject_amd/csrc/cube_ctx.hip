@@ -25,12 +25,42 @@ extern "C" const char *cube_strerror(int code) {
   return "unknown error";
 }
 
+__global__ void k_slots_zero(unsigned long long *s, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s[i] = 0ull;
+}
+__global__ __launch_bounds__(CUBE_SLOTS) void k_slots_fold(const unsigned long long *s, int add,
+                                                          unsigned long long *d64, unsigned *d32) {
+  __shared__ unsigned long long r[CUBE_SLOTS];
+  const int t = threadIdx.x, reg = blockIdx.x;
+  r[t] = s[reg * CUBE_SLOTS + t];
+  __syncthreads();
+  for (int o = CUBE_SLOTS / 2; o > 0; o >>= 1) {
+    if (t < o) r[t] = add ? r[t] + r[t + o] : (r[t] > r[t + o] ? r[t] : r[t + o]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (d64) d64[reg] = r[0];
+    if (d32) d32[reg] = (unsigned)r[0];
+  }
+}
+int cube_slots_zero(cube_ctx *c, unsigned long long *region, int nreg) {
+  hipLaunchKernelGGL(k_slots_zero, dim3(1), dim3(256), 0, c->stream, region, nreg * CUBE_SLOTS);
+  CUBE_LAUNCH_CHECK();
+  return CUBE_OK;
+}
+int cube_slots_fold(cube_ctx *c, const unsigned long long *region, int nreg, int add, unsigned long long *d64,
+                    unsigned *d32) {
+  hipLaunchKernelGGL(k_slots_fold, dim3(nreg), dim3(CUBE_SLOTS), 0, c->stream, region, add, d64, d32);
+  CUBE_LAUNCH_CHECK();
+  return CUBE_OK;
+}
+
 static void free_all(cube_ctx *c) {
   void *ptrs[] = {c->xp, c->vp, c->xp2, c->vp2, c->pid, c->pid2, c->rhoc, c->rhoc2, c->vfield, c->vfield2,
                   c->cstart, c->cstart2, c->dest, c->key, c->sortbuf, c->fill, c->xdest, c->xkey, c->rank, c->xrank, c->fbflag,
                   c->tanf_tab, c->venc_tab, c->vdec0, c->vdec1, c->rho_f, c->spec_f, c->specd_f, c->force_f,
                   c->kern_f, c->rho_c, c->spec_c, c->specd_c, c->force_c, c->kern_c, c->dred,
-                  c->ured, c->ired, c->scanbuf, c->vpart};
+                  c->ured, c->ired, c->scanbuf, c->vpart, c->slots};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   cube_fft_destroy(c->plan_f);
@@ -152,6 +182,7 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->kern_c, 3 * (c->nc / 2 + 1) * c->nc * c->nc); /* unpadded [d][kz][ky][kx] */
   st |= dalloc(&c->dred, 8192);
   st |= dalloc(&c->ured, 64);
+  st |= dalloc(&c->slots, CUBE_NSLOTREG * CUBE_SLOTS);
   st |= dalloc(&c->ired, 4096);
   st |= dalloc(&c->scanbuf, c->ncells / 1024 + 4);
   st |= dalloc(&c->vpart, 3 * (c->ncells / 64 + 2));
@@ -375,7 +406,7 @@ static int pm_forces(cube_ctx *c, float mass_p, hipStream_t fs, hipStream_t cs) 
     st = cube_fft_r2c_3d(c, c->plan_c, c->rho_c, c->spec_c);
     if (!st)
       st = cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc,
-                                 nullptr, -1, (unsigned *)c->ured + 1);
+                                 nullptr, -1, (unsigned *)c->ured + 1, c->slots + CUBE_SLOT_F2_COARSE * CUBE_SLOTS);
     tend(c, CUBE_T_COARSE_FORCE);
   }
   if (!st && hipEventRecord(c->ev_join2, cs) != hipSuccess) st = CUBE_ERR_HIP;
@@ -444,7 +475,8 @@ extern "C" int cube_pm_fine_force(cube_ctx *c, float *f2) {
   if (st) return st;
   tend(c, CUBE_T_FINE_FFT);
   return cube_fft_green_c2r_3d(c, c->plan_f, c->spec_f, c->kern_f, 1, c->specd_f, c->force_f, (float)c->nf, f2,
-                               CUBE_T_FINE_GREEN, (unsigned *)c->ured);
+                               CUBE_T_FINE_GREEN, (unsigned *)c->ured,
+                               c->slots + CUBE_SLOT_F2_FINE * CUBE_SLOTS);
 }
 extern "C" int cube_pm_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new) {
   if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
@@ -475,7 +507,7 @@ extern "C" int cube_pm_coarse_force(cube_ctx *c, float *f2) {
   int st = cube_fft_r2c_3d(c, c->plan_c, c->rho_c, c->spec_c);
   if (st) return st;
   st = cube_fft_green_c2r_3d(c, c->plan_c, c->spec_c, c->kern_c, 0, c->specd_c, c->force_c, (float)c->nc, f2, -1,
-                             (unsigned *)c->ured + 1);
+                             (unsigned *)c->ured + 1, c->slots + CUBE_SLOT_F2_COARSE * CUBE_SLOTS);
   tend(c, CUBE_T_COARSE_FORCE);
   return st;
 }

@@ -263,7 +263,7 @@ static int force_of(cube_dpm *d, const float *x) {
   LAUNCH(k_dpm_deposit, x, d->np, nf, d->rho);
   if ((st = cube_fft_r2c_3d(&d->c, d->plan_f, d->rho, d->spec_f))) return st;
   return cube_fft_green_c2r_3d(&d->c, d->plan_f, d->spec_f, d->kern, 0, d->work3_f, d->force, (float)nf, nullptr,
-                               -1, (unsigned *)d->c.ured);
+                               -1, (unsigned *)d->c.ured, d->c.slots);
 }
 
 /* one KDK step on (x, pm) */
@@ -334,7 +334,7 @@ static int dalloc_f(float **p, int64_t n) {
 extern "C" void cube_dpm_destroy(cube_dpm *d) {
   if (!d) return;
   void *ptrs[] = {d->amp, d->kern, d->xs, d->x, d->pm, d->p0, d->rho, d->force, d->S, d->lam, d->spec_f,
-                  d->work3_f, d->spec_g, d->work3_g, d->psi, d->xbar, d->pbar, d->gvec, d->c.ured};
+                  d->work3_f, d->spec_g, d->work3_g, d->psi, d->xbar, d->pbar, d->gvec, d->c.ured, d->c.slots};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   cube_fft_destroy(d->plan_g);
@@ -361,6 +361,8 @@ extern "C" int cube_dpm_create(const cube_dpm_params *p, cube_dpm **out) {
   int st = d->steps ? CUBE_OK : CUBE_ERR_ALLOC;
   if (!st && hipStreamCreate(&d->c.stream) != hipSuccess) st = CUBE_ERR_HIP;
   if (!st && hipMalloc((void **)&d->c.ured, 64 * sizeof(uint64_t)) != hipSuccess) st = CUBE_ERR_ALLOC;
+  if (!st && hipMalloc((void **)&d->c.slots, CUBE_NSLOTREG * CUBE_SLOTS * sizeof(uint64_t)) != hipSuccess)
+    st = CUBE_ERR_ALLOC;
   if (!st) st = cube_fft_create(&d->c, p->ng, &d->plan_g);
   if (!st) st = cube_fft_create(&d->c, p->nf, &d->plan_f);
   const int64_t np3 = d->np * 3;

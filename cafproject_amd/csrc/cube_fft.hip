@@ -641,12 +641,12 @@ __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ sp
 }
 
 /* half spectra rows -> real rows for the three components, scaled by
-   three divisions by `div`; F written as 3 floats per cell; f2max gets the
-   max of (Fx^2+Fy^2)+Fz^2 as float bits. */
+   three divisions by `div`; F written as 3 floats per cell; the slot region
+   f2slots gets the max of (Fx^2+Fy^2)+Fz^2 as float bits. */
 template <int N>
 __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ specd, float *__restrict__ force,
                                                 const float2 *__restrict__ tw, int64_t nrows, float div,
-                                                unsigned int *__restrict__ f2max) {
+                                                unsigned long long *__restrict__ f2slots) {
   /* half the rows of the single-component kernels: three components in LDS
      at 2 rows per workgroup keep 6 workgroups per CU, which measures faster
      than 4 rows at 3 workgroups per CU despite idle threads in the passes,
@@ -733,7 +733,7 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
     if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
     __syncthreads();
   }
-  if (tid == 0) atomicMax(f2max, __float_as_uint(red[0]));
+  if (tid == 0) atomicMax(cube_slot(f2slots), (unsigned long long)__float_as_uint(red[0]));
 }
 
 /* single-component c2r (no Green's function), for inverse_full */
@@ -765,10 +765,6 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x1(const float2 *__restrict__ spe
     const float2 z = sm[l * LP + (x >> 1)];
     if (row0 + l < nrows) out[(row0 + l) * N + x] = (x & 1) ? z.y : z.x;
   }
-}
-
-__global__ void k_zero_u32(unsigned *p, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
 /* ---------------- host side ---------------- */
@@ -850,7 +846,7 @@ template <int N> struct Launch {
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
   }
-  static int c2r3(cube_ctx *c, FFTPlan *p, const float2 *work3, float *force, float div, unsigned *f2) {
+  static int c2r3(cube_ctx *c, FFTPlan *p, const float2 *work3, float *force, float div, unsigned long long *f2) {
     constexpr int BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC;
     const int64_t rows = (int64_t)N * N;
     hipLaunchKernelGGL(k_c2r_x3<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0, c->stream, work3,
@@ -913,10 +909,12 @@ int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out) {
 /* spec: output of cube_fft_r2c_3d (x and y transformed).  Produces the
    three force components (3 floats per cell) and max |F|^2. */
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
-                          float2 *work3, float *force3, float div, float *f2max, int t0, unsigned *f2dev) {
-  /* max |F|^2 as float bits in the device word f2dev */
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, f2dev, 1);
-  CUBE_LAUNCH_CHECK();
+                          float2 *work3, float *force3, float div, float *f2max, int t0, unsigned *f2dev,
+                          unsigned long long *f2slots) {
+  /* max |F|^2 as float bits: per workgroup into the slots, folded into the
+     device word f2dev */
+  int sst = cube_slots_zero(c, f2slots, 1);
+  if (sst) return sst;
   const int64_t plane = (int64_t)cube_spec_pitch(p->n) * p->n * p->n;
   switch (p->n) {
 #define CASE(N)                                                                   \
@@ -927,7 +925,7 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
     if (t0 >= 0) { tend(c, t0); tbeg(c); }                                        \
     if ((st = Launch<N>::c2c_y<+1>(c, p, work3, 3))) return st;                   \
     if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                    \
-    if ((st = Launch<N>::c2r3(c, p, work3, force3, div, f2dev))) return st;       \
+    if ((st = Launch<N>::c2r3(c, p, work3, force3, div, f2slots))) return st;     \
     if (t0 >= 0) tend(c, t0 + 2);                                                 \
     break;                                                                        \
   }
@@ -936,6 +934,7 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
     default:
       return CUBE_ERR_INVALID;
   }
+  if ((sst = cube_slots_fold(c, f2slots, 1, 0, nullptr, f2dev))) return sst;
   if (!f2max) return CUBE_OK; /* no readback: stays asynchronous (and capturable) */
   unsigned bits = 0;
   CUBE_CHECK(hipMemcpyAsync(&bits, f2dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));

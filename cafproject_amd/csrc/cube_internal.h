@@ -99,6 +99,7 @@ struct cube_ctx {
   hipEvent_t ev_fork, ev_join, ev_join2;
   int pm_ahead;
   float pm_mass;
+  unsigned long long *slots; /* CUBE_NSLOTREG regions of CUBE_SLOTS words, see cube_slots_fold */
   /* stage timers: event pairs recorded without waiting, resolved when the
      ring is full or the timers are read */
   int profiling;
@@ -229,6 +230,22 @@ __host__ __device__ constexpr int64_t cube_kernf_index(int64_t ho, int d, int kz
   return (((d * ho + ky) * cube_kernf_blocks(ho) + (kx >> 4)) * ho + kz) * 16 + (kx & 15);
 }
 
+/* Maxima and sums over a launch.  Atomics of many workgroups on one word
+   serialise in the L2 (about 11 ns each: half a million workgroups wait
+   about 6 ms), so a workgroup updates word blockIdx.x % CUBE_SLOTS of a
+   region instead and cube_slots_fold folds the region afterwards (max and
+   sum do not depend on the order). */
+#define CUBE_SLOTS 256
+enum { CUBE_SLOT_F2_FINE = 0, CUBE_SLOT_F2_COARSE = 1, CUBE_SLOT_PP = 2 /* f2, count */, CUBE_SLOT_VMAX = 4 /* x, y, z */,
+       CUBE_NSLOTREG = 8 };
+__device__ __forceinline__ unsigned long long *cube_slot(unsigned long long *region) {
+  return region + blockIdx.x % CUBE_SLOTS;
+}
+int cube_slots_zero(cube_ctx *c, unsigned long long *region, int nreg);
+/* region r folded into d64[r] (or the low 32 bits into d32[r]): max, or sum when add */
+int cube_slots_fold(cube_ctx *c, const unsigned long long *region, int nreg, int add, unsigned long long *d64,
+                    unsigned *d32);
+
 /* launchers (one file each) */
 int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi, cube_vstats *out);
 int cube_vdec_table(cube_ctx *c, double *dst, float sigma);
@@ -240,7 +257,7 @@ void cube_fft_destroy(FFTPlan *p);
 int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
                           float2 *work3, float *force3, float scale_div, float *f2max, int timer0,
-                          unsigned *f2dev);
+                          unsigned *f2dev, unsigned long long *f2slots);
 int cube_fft_forward_full(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out);
 

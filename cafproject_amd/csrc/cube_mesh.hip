@@ -636,7 +636,8 @@ __device__ __forceinline__ void kick_particle(const int32_t code[3], const int l
 }
 
 /* max of |v + v_c| over the workgroup: wave shuffles, then the waves
-   through LDS, one atomic per component (W/pm.f90:349) */
+   through LDS, one atomic per component into its slot region of
+   vmax_bits (W/pm.f90:349) */
 __device__ __forceinline__ void vmax_reduce(double vm[3], unsigned long long *__restrict__ vmax_bits) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
@@ -649,7 +650,7 @@ __device__ __forceinline__ void vmax_reduce(double vm[3], unsigned long long *__
   if (threadIdx.x < 3) {
     double m = red[threadIdx.x][0];
     for (int w = 1; w < KT / 64; ++w) m = fmax(m, red[threadIdx.x][w]);
-    atomicMax(vmax_bits + threadIdx.x, (unsigned long long)__double_as_longlong(m));
+    atomicMax(cube_slot(vmax_bits + threadIdx.x * CUBE_SLOTS), (unsigned long long)__double_as_longlong(m));
   }
 }
 
@@ -747,10 +748,6 @@ int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sig
   return CUBE_OK;
 }
 
-__global__ void k_zero_u64(unsigned long long *p, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
-}
-
 /* The step's two kicks in one pass over the particles (W/pm.f90:129-159,
    then :193 and :322-387): the fine kick decodes with sigma_vi and encodes
    with sigma_vi_new, the coarse kick decodes and encodes with sigma_vi_new;
@@ -760,15 +757,15 @@ int cube_pm_kicks(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigm
   int st = cube_vdec_table(c, c->vdec0, sigma_vi);
   if (!st) st = cube_vdec_table(c, c->vdec1, sigma_vi_new);
   if (st) return st;
-  unsigned long long *vmb = (unsigned long long *)c->ured + 8;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, vmb, 3);
-  CUBE_LAUNCH_CHECK();
+  unsigned long long *vmb = (unsigned long long *)c->ured + 8, *vsl = c->slots + CUBE_SLOT_VMAX * CUBE_SLOTS;
+  if ((st = cube_slots_zero(c, vsl, 3))) return st;
   float *G = reinterpret_cast<float *>(c->specd_c);
   hipLaunchKernelGGL(k_kick_prescale, dim3(2048), dim3(256), 0, c->stream, c->force_c, 3 * c->ncells, a_mid, dt, G);
   CUBE_LAUNCH_CHECK();
   const double venc = cube_vconst(sigma_vi_new);
-  st = kick<true, 2>(c, c->force_f, a_mid, dt, venc, vmb, G, venc);
+  st = kick<true, 2>(c, c->force_f, a_mid, dt, venc, vsl, G, venc);
   if (st) return st;
+  if ((st = cube_slots_fold(c, vsl, 3, 0, vmb, nullptr))) return st;
   unsigned long long h[3];
   CUBE_CHECK(hipMemcpyAsync(h, vmb, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   CUBE_CHECK(hipStreamSynchronize(c->stream));
@@ -784,15 +781,15 @@ int cube_pm_kicks(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigm
 int cube_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax) {
   int st = cube_vdec_table(c, c->vdec0, sigma_vi);
   if (st) return st;
-  unsigned long long *vmb = (unsigned long long *)c->ured + 8;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, vmb, 3);
-  CUBE_LAUNCH_CHECK();
+  unsigned long long *vmb = (unsigned long long *)c->ured + 8, *vsl = c->slots + CUBE_SLOT_VMAX * CUBE_SLOTS;
+  if ((st = cube_slots_zero(c, vsl, 3))) return st;
   /* G in the coarse spectra scratch, free once force_c is formed */
   float *G = reinterpret_cast<float *>(c->specd_c);
   hipLaunchKernelGGL(k_kick_prescale, dim3(2048), dim3(256), 0, c->stream, c->force_c, 3 * c->ncells, a_mid, dt, G);
   CUBE_LAUNCH_CHECK();
-  st = kick<false>(c, G, a_mid, dt, cube_vconst(sigma_vi), vmb);
+  st = kick<false>(c, G, a_mid, dt, cube_vconst(sigma_vi), vsl);
   if (st) return st;
+  if ((st = cube_slots_fold(c, vsl, 3, 0, vmb, nullptr))) return st;
   unsigned long long h[3];
   CUBE_CHECK(hipMemcpyAsync(h, vmb, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   CUBE_CHECK(hipStreamSynchronize(c->stream));

@@ -123,8 +123,8 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
-    atomicAdd(out + 1, cnt[0]);
+    atomicMax(cube_slot(out), (unsigned long long)__double_as_longlong(red[0]));
+    atomicAdd(cube_slot(out + CUBE_SLOTS), cnt[0]);
   }
 }
 
@@ -417,13 +417,9 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
       f2m = fmax(f2m, wf2[w]);
       ntest += wn[w];
     }
-    atomicMax(out, (unsigned long long)__double_as_longlong(f2m));
-    atomicAdd(out + 1, ntest);
+    atomicMax(cube_slot(out), (unsigned long long)__double_as_longlong(f2m));
+    atomicAdd(cube_slot(out + CUBE_SLOTS), ntest);
   }
-}
-
-__global__ void k_zero_u64_pp(unsigned long long *p, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
 }
 
 template <int IX, int IV>
@@ -460,15 +456,16 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
 int cube_pp(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, float *f2_max_pp, int64_t *itest1) {
   int st = cube_vdec_table(c, c->vdec0, sigma_vi);
   if (st) return st;
-  unsigned long long *out = (unsigned long long *)c->ured + 16;
-  hipLaunchKernelGGL(k_zero_u64_pp, dim3(1), dim3(64), 0, c->stream, out, 2);
-  CUBE_LAUNCH_CHECK();
+  unsigned long long *out = (unsigned long long *)c->ured + 16, *osl = c->slots + CUBE_SLOT_PP * CUBE_SLOTS;
+  if ((st = cube_slots_zero(c, osl, 2))) return st;
   const int x = c->p.izipx, v = c->p.izipv;
-  if (x == 1 && v == 1) st = pp_impl<1, 1>(c, a_mid, dt, sigma_vi, mass_p, out);
-  else if (x == 1) st = pp_impl<1, 2>(c, a_mid, dt, sigma_vi, mass_p, out);
-  else if (v == 1) st = pp_impl<2, 1>(c, a_mid, dt, sigma_vi, mass_p, out);
-  else st = pp_impl<2, 2>(c, a_mid, dt, sigma_vi, mass_p, out);
+  if (x == 1 && v == 1) st = pp_impl<1, 1>(c, a_mid, dt, sigma_vi, mass_p, osl);
+  else if (x == 1) st = pp_impl<1, 2>(c, a_mid, dt, sigma_vi, mass_p, osl);
+  else if (v == 1) st = pp_impl<2, 1>(c, a_mid, dt, sigma_vi, mass_p, osl);
+  else st = pp_impl<2, 2>(c, a_mid, dt, sigma_vi, mass_p, osl);
   if (st) return st;
+  if ((st = cube_slots_fold(c, osl, 1, 0, out, nullptr)) || (st = cube_slots_fold(c, osl + CUBE_SLOTS, 1, 1, out + 1, nullptr)))
+    return st;
   unsigned long long h[2];
   CUBE_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   CUBE_CHECK(hipStreamSynchronize(c->stream));
