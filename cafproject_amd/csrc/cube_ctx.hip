@@ -381,7 +381,7 @@ extern "C" int cube_buffer_v(cube_ctx *c) {
    ev_join and ev_join2 mark their ends.  Nothing is read back: the mesh sums
    stay in dred and max |F|^2 in ured words 0 (fine) and 1 (coarse).  The
    two pipelines read only the positions and write disjoint buffers. */
-static int pm_forces(cube_ctx *c, float mass_p, hipStream_t fs, hipStream_t cs) {
+static int pm_forces(cube_ctx *c, float mass_p, float a_mid, float dt, hipStream_t fs, hipStream_t cs) {
   const hipStream_t main = c->stream;
   CUBE_CHECK(hipEventRecord(c->ev_fork, main));
   int st = CUBE_OK;
@@ -392,7 +392,17 @@ static int pm_forces(cube_ctx *c, float mass_p, hipStream_t fs, hipStream_t cs) 
     st = cube_fine_deposit(c, mass_p, nullptr);
     tend(c, CUBE_T_FINE_DEPOSIT);
   }
-  if (!st) st = cube_pm_fine_force(c, nullptr);
+  if (!st) {
+    /* force_f gets the fine kick's per-term factors for this a_mid and dt */
+    const float pre[2] = {a_mid, dt};
+    tbeg(c);
+    st = cube_fft_r2c_3d(c, c->plan_f, c->rho_f, c->spec_f);
+    tend(c, CUBE_T_FINE_FFT);
+    if (!st)
+      st = cube_fft_green_c2r_3d(c, c->plan_f, c->spec_f, c->kern_f, 1, c->specd_f, c->force_f, (float)c->nf,
+                                 nullptr, CUBE_T_FINE_GREEN, (unsigned *)c->ured,
+                                 c->slots + CUBE_SLOT_F2_FINE * CUBE_SLOTS, pre);
+  }
   if (!st && hipEventRecord(c->ev_join, fs) != hipSuccess) st = CUBE_ERR_HIP;
   c->stream = cs;
   if (!st && hipStreamWaitEvent(cs, c->ev_fork, 0) != hipSuccess) st = CUBE_ERR_HIP;
@@ -435,7 +445,7 @@ extern "C" int cube_pp_force(cube_ctx *c, float a_mid, float dt, float sigma_vi,
      has changed the state in between and mass_p agrees. */
   c->pm_ahead = 0;
   if (c->has_kernels) {
-    st = pm_forces(c, mass_p, c->side, c->side2);
+    st = pm_forces(c, mass_p, a_mid, dt, c->side, c->side2);
     if (st) return st;
   }
   int64_t itest1 = 0;
@@ -451,6 +461,8 @@ extern "C" int cube_pp_force(cube_ctx *c, float a_mid, float dt, float sigma_vi,
   if (c->has_kernels) {
     c->pm_ahead = 1;
     c->pm_mass = mass_p;
+    c->pm_amid = a_mid;
+    c->pm_dt = dt;
   }
   return CUBE_OK;
 }
@@ -536,9 +548,9 @@ extern "C" int cube_particle_mesh(cube_ctx *c, float a_mid, float dt, float sigm
      already formed both beside PP); the two kicks, which both rewrite the
      velocities, stay in the reference's order on the main stream
      (W/pm.f90:2-415).  Scalars are read back at the end. */
-  const int ahead = c->pm_ahead && c->pm_mass == mass_p;
+  const int ahead = c->pm_ahead && c->pm_mass == mass_p && c->pm_amid == a_mid && c->pm_dt == dt;
   c->pm_ahead = 0;
-  if (!ahead && (st = pm_forces(c, mass_p, c->stream, c->side))) return st;
+  if (!ahead && (st = pm_forces(c, mass_p, a_mid, dt, c->stream, c->side))) return st;
   CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   CUBE_CHECK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
   /* the fine kick, sigma_vi = sigma_vi_new (W/pm.f90:193), the coarse kick:

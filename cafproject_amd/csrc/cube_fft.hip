@@ -646,7 +646,8 @@ __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ sp
 template <int N>
 __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ specd, float *__restrict__ force,
                                                 const float2 *__restrict__ tw, int64_t nrows, float div,
-                                                unsigned long long *__restrict__ f2slots) {
+                                                unsigned long long *__restrict__ f2slots, float pa, float pdt,
+                                                int pre) {
   /* half the rows of the single-component kernels: three components in LDS
      at 2 rows per workgroup keep 6 workgroups per CU, which measures faster
      than 4 rows at 3 workgroups per CU despite idle threads in the passes,
@@ -722,7 +723,21 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
       struct F3 {
         float x, y, z;
       };
-      *reinterpret_cast<F3 *>(force + ((row0 + l) * N + x) * 3) = F3{f[0], f[1], f[2]}; /* one 12-byte store */
+      float o[3] = {f[0], f[1], f[2]};
+      if (pre) {
+        /* the kick's factor of each term, (((F*a_mid)*dt)/6)/pi
+           (W/pm.f90:146-153), formed once per mesh value instead of once
+           per particle and corner; the IEEE quotient where div_6pi's
+           range check fails */
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float a = (f[d] * pa) * pdt;
+          bool ok = true;
+          const float gq = div_6pi(a, ok);
+          o[d] = ok ? gq : (a / 6.0f) / CUBE_PI_F;
+        }
+      }
+      *reinterpret_cast<F3 *>(force + ((row0 + l) * N + x) * 3) = F3{o[0], o[1], o[2]}; /* one 12-byte store */
       const float f2 = (f[0] * f[0] + f[1] * f[1]) + f[2] * f[2];
       m = fmaxf(m, f2);
     }
@@ -846,11 +861,12 @@ template <int N> struct Launch {
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
   }
-  static int c2r3(cube_ctx *c, FFTPlan *p, const float2 *work3, float *force, float div, unsigned long long *f2) {
+  static int c2r3(cube_ctx *c, FFTPlan *p, const float2 *work3, float *force, float div, unsigned long long *f2,
+                  float pa, float pdt, int pre) {
     constexpr int BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC;
     const int64_t rows = (int64_t)N * N;
     hipLaunchKernelGGL(k_c2r_x3<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0, c->stream, work3,
-                       force, p->tw, rows, div, f2);
+                       force, p->tw, rows, div, f2, pa, pdt, pre);
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
   }
@@ -907,10 +923,13 @@ int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out) {
 }
 
 /* spec: output of cube_fft_r2c_3d (x and y transformed).  Produces the
-   three force components (3 floats per cell) and max |F|^2. */
+   three force components (3 floats per cell) and max |F|^2; with prescale
+   = {a_mid, dt} the cells hold the fine kick's (((F*a_mid)*dt)/6)/pi
+   instead (max |F|^2 still of F). */
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
                           float2 *work3, float *force3, float div, float *f2max, int t0, unsigned *f2dev,
-                          unsigned long long *f2slots) {
+                          unsigned long long *f2slots, const float *prescale) {
+  const float pa = prescale ? prescale[0] : 0.f, pdt = prescale ? prescale[1] : 0.f;
   /* max |F|^2 as float bits: per workgroup into the slots, folded into the
      device word f2dev */
   int sst = cube_slots_zero(c, f2slots, 1);
@@ -925,7 +944,7 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
     if (t0 >= 0) { tend(c, t0); tbeg(c); }                                        \
     if ((st = Launch<N>::c2c_y<+1>(c, p, work3, 3))) return st;                   \
     if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                    \
-    if ((st = Launch<N>::c2r3(c, p, work3, force3, div, f2slots))) return st;     \
+    if ((st = Launch<N>::c2r3(c, p, work3, force3, div, f2slots, pa, pdt, prescale != nullptr))) return st; \
     if (t0 >= 0) tend(c, t0 + 2);                                                 \
     break;                                                                        \
   }

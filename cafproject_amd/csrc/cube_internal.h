@@ -98,7 +98,7 @@ struct cube_ctx {
   hipStream_t side, side2;
   hipEvent_t ev_fork, ev_join, ev_join2;
   int pm_ahead;
-  float pm_mass;
+  float pm_mass, pm_amid, pm_dt; /* force_f then holds the kick factors of this a_mid, dt */
   unsigned long long *slots; /* CUBE_NSLOTREG regions of CUBE_SLOTS words, see cube_slots_fold */
   /* stage timers: event pairs recorded without waiting, resolved when the
      ring is full or the timers are read */
@@ -213,6 +213,33 @@ __device__ __forceinline__ int32_t cube_encode_v_tab(double c, double v, const d
   return cube_wrap(IV, u < 0 ? -k : k);
 }
 
+/* a/b for a divisor fixed at compile time, y = RN(1/b): q0 = RN(a*y) and
+   Markstein's correction RN(q0 + (a - b*q0)*y) with both steps fused.  For
+   b = 6 and b = pi in real(4) this equals the IEEE quotient bit for bit for
+   every 1e-30 <= |a| < 1e30 (checked on the CPU over every float of that
+   range by tests/test_divide_const.py). */
+template <int B> struct FixedDiv;
+template <> struct FixedDiv<6> {
+  static constexpr float b = 6.0f, y = 1.0f / 6.0f;
+};
+template <> struct FixedDiv<314> {
+  static constexpr float b = CUBE_PI_F, y = 1.0f / CUBE_PI_F;
+};
+template <int B> __device__ __forceinline__ float div_fixed(float a) {
+  constexpr float b = FixedDiv<B>::b, y = FixedDiv<B>::y;
+  const float q0 = a * y;
+  const float r = __builtin_fmaf(-q0, b, a);
+  return __builtin_fmaf(r, y, q0);
+}
+/* ((a/6)/pi) for the kicks: when 6e-30 <= |a| < 1e30 both quotients stay in
+   the checked range (a/6 lands in [1e-30, 1.7e29)); `ok` is cleared
+   otherwise and the caller redoes the sum with divisions. */
+__device__ __forceinline__ float div_6pi(float a, bool &ok) {
+  const float m = fabsf(a);
+  ok = ok && m >= 6e-30f && m < 1e30f;
+  return div_fixed<314>(div_fixed<6>(a));
+}
+
 /* Half spectra of an n^3 real mesh are stored as rows of n/2+1 complex
    values padded to a multiple of 16 (128 bytes), so every row and every
    16-column batch of the strided FFT passes starts on a cache line. */
@@ -257,7 +284,7 @@ void cube_fft_destroy(FFTPlan *p);
 int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
                           float2 *work3, float *force3, float scale_div, float *f2max, int timer0,
-                          unsigned *f2dev, unsigned long long *f2slots);
+                          unsigned *f2dev, unsigned long long *f2slots, const float *prescale = nullptr);
 int cube_fft_forward_full(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out);
 

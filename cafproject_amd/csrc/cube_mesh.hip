@@ -529,33 +529,6 @@ struct F3 {
   float x, y, z;
 };
 
-/* a/b for a divisor fixed at compile time, y = RN(1/b): q0 = RN(a*y) and
-   Markstein's correction RN(q0 + (a - b*q0)*y) with both steps fused.  For
-   b = 6 and b = pi in real(4) this equals the IEEE quotient bit for bit for
-   every 1e-30 <= |a| < 1e30 (checked on the CPU over every float of that
-   range by tests/test_divide_const.py). */
-template <int B> struct FixedDiv;
-template <> struct FixedDiv<6> {
-  static constexpr float b = 6.0f, y = 1.0f / 6.0f;
-};
-template <> struct FixedDiv<314> {
-  static constexpr float b = CUBE_PI_F, y = 1.0f / CUBE_PI_F;
-};
-template <int B> __device__ __forceinline__ float div_fixed(float a) {
-  constexpr float b = FixedDiv<B>::b, y = FixedDiv<B>::y;
-  const float q0 = a * y;
-  const float r = __builtin_fmaf(-q0, b, a);
-  return __builtin_fmaf(r, y, q0);
-}
-/* ((a/6)/pi) for the kicks: when 6e-30 <= |a| < 1e30 both quotients stay in
-   the checked range (a/6 lands in [1e-30, 1.7e29)); `ok` is cleared
-   otherwise and the caller redoes the sum with divisions. */
-__device__ __forceinline__ float div_6pi(float a, bool &ok) {
-  const float m = fabsf(a);
-  ok = ok && m >= 6e-30f && m < 1e30f;
-  return div_fixed<314>(div_fixed<6>(a));
-}
-
 /* G = (((F*a_mid)*dt)/6)/pi per mesh value, the factor of every kick term
    (W/pm.f90:341-348); the coarse kick reads G so each coarse cell's value is
    formed once instead of by every particle whose cloud touches it */
@@ -572,10 +545,12 @@ __global__ void k_kick_prescale(const float *__restrict__ F, int64_t n, float a_
    the 8 corners in the order of W/pm.f90:146-153 (fine: force_f with
    ((((F*a_mid)*dt)/6)/pi)*wx*wy*wz per term; coarse: the prescaled G with
    G*wx*wy*wz, W/pm.f90:341-348).  loc: 1-based cell in the tile, tt: tile. */
-template <int IX, bool FINE>
+template <int IX, bool FINE, bool PRE = false>
 __device__ __forceinline__ void kick_particle(const int32_t code[3], const int loc[3], const int tt[3],
                                               const Geo &g, const float *__restrict__ force, float a_mid,
                                               float dt, double vreal[3]) {
+  /* PRE: the fine mesh already holds (((F*a_mid)*dt)/6)/pi per value */
+  constexpr bool DIV = FINE && !PRE;
   const int nt = g.nt, nfine = 4 * g.nc, nft = 4 * nt;
   int i1[3];
   float d1[3], d2[3];
@@ -615,12 +590,12 @@ __device__ __forceinline__ void kick_particle(const int32_t code[3], const int l
     const float F[3] = {Fv.x, Fv.y, Fv.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const float gq = FINE ? div_6pi((F[a] * a_mid) * dt, ok) : F[a];
+      const float gq = DIV ? div_6pi((F[a] * a_mid) * dt, ok) : F[a];
       const float inc = ((gq * wx) * wy) * wz;
       vreal[a] = vreal[a] + (double)inc;
     }
   }
-  if (FINE && !ok) { /* a quotient outside the checked range: redo with divisions */
+  if (DIV && !ok) { /* a quotient outside the checked range: redo with divisions */
 #pragma unroll
     for (int a = 0; a < 3; ++a) vreal[a] = v0[a];
     for (int q = 0; q < 8; ++q) {
@@ -657,7 +632,8 @@ __device__ __forceinline__ void vmax_reduce(double vm[3], unsigned long long *__
 /* kicks, one workgroup per cell row (tile, k, j): particles of the row are
    contiguous; each thread finds its cell by binary search in LDS.
    NKICK = 1: the one kick of FINE (decode with vdec, encode with venc);
-   NKICK = 2: the fine kick (vdec, venc), then the coarse one on the
+   NKICK = 2: the fine kick (vdec, venc) from the prescaled fine mesh that
+   the step's force pipeline leaves in force_f, then the coarse one on the
    re-decoded code (vdec2, venc2) in the same pass, which is the step's
    sequence W/pm.f90:129-159 then 322-354 with the intermediate codes kept
    in registers instead of going through vp. */
@@ -695,7 +671,7 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
       code[a] = (int32_t)xp[3 * p + a];
       vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
     }
-    kick_particle<IX, FINE>(code, loc, tt, g, force, a_mid, dt, vreal);
+    kick_particle<IX, FINE, FINE && NKICK == 2>(code, loc, tt, g, force, a_mid, dt, vreal);
     if (NKICK == 2) {
       /* the code the fine kick would store, decoded again for the coarse kick */
 #pragma unroll
@@ -751,8 +727,10 @@ int cube_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sig
 /* The step's two kicks in one pass over the particles (W/pm.f90:129-159,
    then :193 and :322-387): the fine kick decodes with sigma_vi and encodes
    with sigma_vi_new, the coarse kick decodes and encodes with sigma_vi_new;
-   vmax (W/pm.f90:349) as cube_coarse_kick.  Equal to cube_fine_kick then
-   cube_coarse_kick(sigma_vi_new) bit for bit. */
+   vmax (W/pm.f90:349) as cube_coarse_kick.  force_f holds the fine kick's
+   factors (((F*a_mid)*dt)/6)/pi, formed by the step's force pipeline for
+   this a_mid and dt.  Equal to cube_fine_kick then
+   cube_coarse_kick(sigma_vi_new) on the plain force bit for bit. */
 int cube_pm_kicks(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new, float *vmax) {
   int st = cube_vdec_table(c, c->vdec0, sigma_vi);
   if (!st) st = cube_vdec_table(c, c->vdec1, sigma_vi_new);

@@ -169,7 +169,10 @@ int cube_set_state_xv(cube_ctx *ctx, int64_t n, const double *x, const float *v,
 /* Mesh fields, for diagnostics and checks.  which: */
 enum {
   CUBE_FIELD_RHO_F = 0,   /* nf^3 float, fine density, (x,y,z) x fastest           */
-  CUBE_FIELD_FORCE_F = 1, /* nf^3 x 3 float, fine force, 3 components per cell      */
+  CUBE_FIELD_FORCE_F = 1, /* nf^3 x 3 float, fine force, 3 components per cell;
+                              after ext_pp_force or particle_mesh it holds the
+                              fine kick's factors (((F*a_mid)*dt)/6)/pi
+                              (cube_pm_fine_force forms F itself)             */
   CUBE_FIELD_RHO_C = 2,   /* nc^3 float, coarse density                             */
   CUBE_FIELD_FORCE_C = 3, /* nc^3 x 3 float, coarse force                           */
   CUBE_FIELD_KERN_F = 4,  /* 3 x (nf/2+1)^3 float, fine Green's function octant     */
