@@ -673,9 +673,11 @@ __global__ __launch_bounds__(1024) void k_sum_parts_tree(const double *__restric
                                                          double *__restrict__ out) {
   __shared__ double red[3][1024];
   const int t = threadIdx.x;
-  const int64_t per = (nb + 1023) / 1024, b0 = t * per, b1 = min(b0 + per, nb);
+  /* thread t sums the partials b = t mod 1024 (a fixed order, so the sums
+     repeat bit for bit; neighbouring lanes read neighbouring partials) */
   double s[3] = {0, 0, 0};
-  for (int64_t b = b0; b < b1; ++b)
+  for (int64_t b = t; b < nb; b += 1024)
+#pragma unroll
     for (int a = 0; a < 3; ++a) s[a] += part[3 * b + a];
   for (int a = 0; a < 3; ++a) red[a][t] = s[a];
   __syncthreads();

@@ -164,12 +164,20 @@ __global__ __launch_bounds__(MT) void k_coarse_deposit(const typename IntOf<IX>:
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_sum_parts1(const double *part, int nb, double *out) {
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int b = 0; b < nb; ++b) s += part[b];
-    *out = s;
+/* sum of nb partials in a fixed order (thread t takes b = t mod 64, then a
+   tree over the threads), so the sum repeats bit for bit */
+__global__ __launch_bounds__(64) void k_sum_parts1(const double *part, int nb, double *out) {
+  __shared__ double r[64];
+  const int t = threadIdx.x;
+  double s = 0;
+  for (int b = t; b < nb; b += 64) s += part[b];
+  r[t] = s;
+  __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    if (t < o) r[t] += r[t + o];
+    __syncthreads();
   }
+  if (t == 0) *out = r[0];
 }
 
 /*
