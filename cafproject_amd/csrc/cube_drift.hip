@@ -144,11 +144,16 @@ int cube_scan_counts(cube_ctx *c, const int32_t *cnt, int64_t *start, int64_t n,
 __global__ void k_tile_boxsum(const int32_t *rhoc, Geo g, int w, unsigned long long *out) {
   const int t = blockIdx.y;
   const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
-  const int64_t e = g.nt + 2 * w, vol = e * e * e;
+  /* (nt+2w)^3 < 2^31 cells per box: 32-bit index arithmetic; the tile's
+     own cells by their id, the buffer shell through cid_global */
+  const int e = g.nt + 2 * w, nt = g.nt;
+  const uint32_t vol = (uint32_t)e * e * e, ue = (uint32_t)e;
   int64_t s = 0;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < vol; q += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(q % e), j = (int)((q / e) % e), k = (int)(q / (e * e));
-    s += rhoc[cid_global(g, tx * g.nt + i - w, ty * g.nt + j - w, tz * g.nt + k - w)];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < vol; q += gridDim.x * blockDim.x) {
+    const uint32_t qe = q / ue;
+    const int i = (int)(q - qe * ue) - w, j = (int)(qe % ue) - w, k = (int)(qe / ue) - w;
+    const bool in = i >= 0 && i < nt && j >= 0 && j < nt && k >= 0 && k < nt;
+    s += rhoc[in ? g.cid(i, j, k, t) : cid_global(g, tx * nt + i, ty * nt + j, tz * nt + k)];
   }
   __shared__ int64_t r[256];
   r[threadIdx.x] = s;
