@@ -112,3 +112,26 @@ def test_plane_waves_1024():
             want += -amp * s[..., None] * _kd(kern, k, N)[None, None, :]
         err = rel_l2(f[z], want)
         assert err < 1e-5, (z, err)
+
+
+@pytest.mark.parametrize("nc", [16, 24])
+def test_kern_f_crosses_the_boundary_as_the_octant(nc):
+    """kern_f is blocked on the device and crosses cube_get_field /
+    cube_set_field as the plain octant.  Setting 2*K must double the force
+    exactly (the solve is linear and scaling by 2 is exact), which fails if
+    the two conversions are not inverse layouts of the device one.  nc=16
+    (nf=64) takes the register z-pass, nc=24 (nf=96) the LDS one."""
+    from cafproject_amd.cube import Cube
+    nf = 4 * nc
+    rho = np.random.default_rng(nc).random((nf, nf, nf), dtype=np.float32)
+    g = Cube(nc, 1, np_nc=1)
+    g.init_kernels()
+    g.set_field("rho_f", rho)
+    g.fine_force()
+    f1 = g.get_field("force_f")
+    k = g.get_field("kern_f")
+    g.set_field("kern_f", 2 * k)
+    assert np.array_equal(g.get_field("kern_f"), 2 * k)
+    g.set_field("rho_f", rho)
+    g.fine_force()
+    assert np.array_equal(g.get_field("force_f"), 2 * f1)
