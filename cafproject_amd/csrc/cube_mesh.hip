@@ -690,10 +690,13 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
       kick_particle<IX, false>(code, loc, tt, g, force2, a_mid, dt, vreal);
     }
     const double ve = NKICK == 2 ? venc2 : venc;
+    F3 vc = {0.f, 0.f, 0.f};
+    if (VMAX) vc = *reinterpret_cast<const F3 *>(vfield + 3 * (c0 + lo)); /* one 12-byte load */
+    const float vcf[3] = {vc.x, vc.y, vc.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (VMAX) {
-        const double av = fabs(vreal[a] + (double)vfield[3 * (c0 + lo) + a]);
+        const double av = fabs(vreal[a] + (double)vcf[a]);
         vm[a] = av > vm[a] ? av : vm[a];
       }
       vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v(IV, ve, vreal[a]);
