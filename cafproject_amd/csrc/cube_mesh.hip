@@ -330,7 +330,8 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
                   for (int a = 0; a < 3; ++a) {
                     /* ncell*(i-1)+ncell*(x offset) rounded to real(4) (W/pm.f90:63):
                        both terms are exact floats, so one float add rounds it */
-                    const float off4 = (float)(4.0 * cube_xoff(IX, cq[u][a]));
+                    /* (u+0.5)*4*2^-8izipx: exact in real(4) as in real(8) */
+                    const float off4 = ((float)cube_ucode(IX, cq[u][a]) + 0.5f) * (IX == 1 ? 0x1p-6f : 0x1p-14f);
                     const float tempx = (float)(4 * (ci[a] - 1)) + off4;
                     const int i1 = (int)floorf(tempx) + 1;
                     d1[a] = (float)i1 - tempx;

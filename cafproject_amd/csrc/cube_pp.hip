@@ -290,12 +290,13 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     int cx = 0;
     while (cx < 3 && cst[row][cx + 1] <= p) ++cx;
     const int loc[3] = {B0[0] + 1 + cx, B0[1] + 1 + row % 4, B0[2] + 1 + row / 4};
-    double x1[3], vreal[3], ft[3] = {0, 0, 0};
+    double vreal[3], ft[3] = {0, 0, 0};
     int X1[3], gf[3];
+    int32_t pcode[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int32_t code = (int32_t)xp[3 * p + a];
-      x1[a] = (double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, code);
+      pcode[a] = code;
       X1[a] = pp_fix(IX, loc[a], code) - X0[a];
       gf[a] = 4 * (loc[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
       vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
@@ -348,6 +349,9 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
         }
       }
     } else {
+    double x1[3]; /* FP64 tile-frame position for the walk over the global lists */
+#pragma unroll
+    for (int a = 0; a < 3; ++a) x1[a] = (double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, pcode[a]);
     for (int kk = gf[2] - pr; kk <= gf[2] + pr; ++kk)
       for (int jj = gf[1] - pr; jj <= gf[1] + pr; ++jj)
         for (int ii = gf[0] - pr; ii <= gf[0] + pr; ++ii) {
