@@ -566,8 +566,9 @@ __device__ __forceinline__ void kick_particle(const int32_t code[3], const int l
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     float tempx;
-    if (FINE) /* ncell*((/i,j,k/)-1)+ncell*(...)*x_resolution, tile frame (W/pm.f90:138) */
-      tempx = (float)((double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, code[a]));
+    if (FINE) /* ncell*((/i,j,k/)-1)+ncell*(...)*x_resolution, tile frame (W/pm.f90:138):
+                 both terms are exact floats, so one float add rounds it */
+      tempx = (float)(4 * (loc[a] - 1)) + ((float)cube_ucode(IX, code[a]) + 0.5f) * (IX == 1 ? 0x1p-6f : 0x1p-14f);
     else /* ((/itx,../)-1)*nt+((/i,../)-1)+(...)*x_resolution-0.5 (W/pm.f90:335) */
       tempx = (float)(((double)(tt[a] * nt + (loc[a] - 1)) + cube_xoff(IX, code[a])) - 0.5);
     i1[a] = (int)floorf(tempx) + 1;
