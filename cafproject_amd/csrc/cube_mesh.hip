@@ -107,6 +107,17 @@ __global__ __launch_bounds__(MT) void k_fine_deposit(const typename IntOf<IX>::T
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+/* the coarse CIC coordinate ((cell-1)+(x offset))-0.5 of W/pm.f90:213,335
+   rounded to real(4); c = cell-1 (tile frame or global).  With 1-byte codes
+   c + (u+0.5)/256 has at most 24 significant bits for c < 2^15, so the
+   real(4) sums are exact and round as the real(8) form does; 2-byte codes
+   keep the real(8) form. */
+template <int IX> __device__ __forceinline__ float coarse_tempx(int c, int32_t code) {
+  if constexpr (IX == 1)
+    return ((float)c + ((float)cube_ucode(IX, code) + 0.5f) * 0x1p-8f) - 0.5f;
+  else
+    return (float)(((double)c + cube_xoff(IX, code)) - 0.5);
+}
 /* ---------------- coarse CIC deposit, W/pm.f90:205-267 ---------------- */
 template <int IX>
 __global__ __launch_bounds__(MT) void k_coarse_deposit(const typename IntOf<IX>::T *__restrict__ xp,
@@ -141,7 +152,7 @@ __global__ __launch_bounds__(MT) void k_coarse_deposit(const typename IntOf<IX>:
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               const int32_t code = (int32_t)xp[3 * (p0 + l) + a];
-              const float tempx = (float)(((double)(ci[a] - 1) + cube_xoff(IX, code)) - 0.5);
+              const float tempx = coarse_tempx<IX>(ci[a] - 1, code);
               const int i1 = (int)floorf(tempx) + 1;
               const float d1 = (float)i1 - tempx;
               if (i1 == lm[a]) w[a] = d1;
@@ -458,7 +469,7 @@ __global__ __launch_bounds__(MT) void k_coarse_deposit_b2(const typename IntOf<I
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               const int32_t code = code4[u][a];
-              const float tempx = (float)(((double)(ci[a] - 1) + cube_xoff(IX, code)) - 0.5);
+              const float tempx = coarse_tempx<IX>(ci[a] - 1, code);
               const int i1 = (int)floorf(tempx) + 1;
               const float d1 = (float)i1 - tempx, d2 = 1.0f - d1;
               const int r = i1 - m0[a]; /* corner 1 relative to the block */
@@ -570,7 +581,7 @@ __device__ __forceinline__ void kick_particle(const int32_t code[3], const int l
                  both terms are exact floats, so one float add rounds it */
       tempx = (float)(4 * (loc[a] - 1)) + ((float)cube_ucode(IX, code[a]) + 0.5f) * (IX == 1 ? 0x1p-6f : 0x1p-14f);
     else /* ((/itx,../)-1)*nt+((/i,../)-1)+(...)*x_resolution-0.5 (W/pm.f90:335) */
-      tempx = (float)(((double)(tt[a] * nt + (loc[a] - 1)) + cube_xoff(IX, code[a])) - 0.5);
+      tempx = coarse_tempx<IX>(tt[a] * nt + (loc[a] - 1), code[a]);
     i1[a] = (int)floorf(tempx) + 1;
     d1[a] = (float)i1[a] - tempx;
     d2[a] = 1.0f - d1[a];
