@@ -68,11 +68,14 @@ DRIVER = os.path.join(FORT, "cube_main")
 
 def build_fortran():
     """The Fortran driver (fortran/cube_main.f90 over the ISO_C_BINDING
-    module fortran/cube_c.f90), linked against libcube_hip.so."""
+    module fortran/cube_c.f90 and the time integration fortran/cube_clock.f90),
+    linked against libcube_hip.so.  No FMA contraction, as in the HIP
+    library."""
     mod = os.path.join(BUILD, "fmod")
     os.makedirs(mod, exist_ok=True)
     tmp = DRIVER + ".tmp"
-    cmd = [FC, "-O2", "-module-dir", mod, os.path.join(FORT, "cube_c.f90"), os.path.join(FORT, "cube_main.f90"),
+    cmd = [FC, "-O2", "-ffp-contract=off", "-module-dir", mod, os.path.join(FORT, "cube_c.f90"),
+           os.path.join(FORT, "cube_clock.f90"), os.path.join(FORT, "cube_main.f90"),
            "-o", tmp, "-L" + HERE, "-lcube_hip", "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

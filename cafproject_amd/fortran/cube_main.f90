@@ -15,6 +15,7 @@
 program cube_main
   use iso_c_binding
   use cube_c
+  use cube_clock
   implicit none
 
   ! sim_header (W/parameters.f90:136-160): 17 integer(8) + 22 real(4)
@@ -220,32 +221,6 @@ contains
     close (u)
   end subroutine
 
-  ! one step of the Friedmann equation: third-order Taylor expansion with
-  ! real(8) internals, a and dt real(4) (W/timestep.f90:133-177)
-  subroutine expansion(a0, dt0, da1, da2)
-    real(4), intent(in) :: a0, dt0
-    real(4), intent(out) :: da1, da2
-    real(8) :: d, ax, adot, addot, atdot
-    d = dble(dt0 / 2.0)
-    call derivs(dble(a0), adot, addot, atdot)
-    da1 = real(adot * d + (addot * d**2) / 2d0 + (atdot * d**3) / 6d0, 4)
-    ax = dble(a0 + da1)
-    call derivs(ax, adot, addot, atdot)
-    da2 = real(adot * d + (addot * d**2) / 2d0 + (atdot * d**3) / 6d0, 4)
-  end subroutine
-
-  subroutine derivs(ax, adot, addot, atdot)
-    real(8), intent(in) :: ax
-    real(8), intent(out) :: adot, addot, atdot
-    real(8), parameter :: omhsq = 4d0 / 9d0, w = -1d0
-    real(8) :: a3rlm, arkm
-    a3rlm = ax**3 * dble(ol) / dble(om)
-    arkm = ax * (1d0 - dble(om) - dble(ol)) / dble(om)
-    adot = sqrt(omhsq * ax**3 * (1d0 + arkm + a3rlm))
-    addot = ax**2 * omhsq * (1.5d0 + 2d0 * arkm + 1.5d0 * (1d0 - w) * a3rlm)
-    atdot = ax * adot * omhsq * (3d0 + 6d0 * arkm + 1.5d0 * (2d0 - 3d0 * w) * (1d0 - w) * a3rlm)
-  end subroutine
-
   ! W/timestep.f90:1-92 for one image, checkpoint landing at z_out
   subroutine timestep()
     real(4) :: dt_e, da1, da2, ra, a_next
@@ -255,7 +230,7 @@ contains
     ntry = 0
     do
       ntry = ntry + 1
-      call expansion(a, dt_e, da1, da2)
+      call expansion(a, dt_e, om, ol, -1.0, da1, da2)
       da = da1 + da2
       ra = da / (a + da)
       if (ra > 0.2) then
@@ -266,14 +241,14 @@ contains
       if (ntry > 10) exit
     end do
     dt = min(dt_e, dt_fine, dt_coarse, dt_pp, dt_vmax)
-    call expansion(a, dt, da1, da2)
+    call expansion(a, dt, om, ol, -1.0, da1, da2)
     da = da1 + da2
     a_next = 1.0 / (1.0 + z_out)
     if (da >= a_next - a) then
       n = 0
-      do while (abs(dble((a + da) / a_next) - 1d0) >= 1d-6 .and. n < 100)
+      do while (abs((a + da) / a_next - 1) >= 1e-6 .and. n < 100)
         dt = dt * (a_next - a) / da
-        call expansion(a, dt, da1, da2)
+        call expansion(a, dt, om, ol, -1.0, da1, da2)
         da = da1 + da2
         n = n + 1
       end do

@@ -2,10 +2,18 @@
 CUBEnu/work/main/timestep.f90:1-92,133-177.
 
 The reference keeps a, da, dt, a_mid in real(4) and evaluates the
-third-order Taylor step of the Friedmann equation with real(8) internals;
-the same kinds are kept here (numpy float32 / Python float), so a run driven
-by this module takes the same (a, dt) sequence as the reference given the
-same dt constraints.
+third-order Taylor step of the Friedmann equation with real(8) internals.
+Each sub-expression here has the kind gfortran gives it there (numpy
+float32 for real(4), Python float for real(8)), so a run driven by this
+module takes the same (a, dt) sequence as the reference given the same dt
+constraints:
+
+  dt_x = dt0/2, dt_x**2, dt_x**3      real(4)  (W/timestep.f90:143,158,175)
+  omHsq = 4.0/9.0                     the real(4) quotient widened (:145,161)
+  a_x**(-3*wde)                       a libm pow call (real exponent, :146,163)
+  a_x**3, a_x**2                      (x*x)*x and x*x (integer powers, :150-156)
+  a_x = a0 + da1                      a real(4) sum widened (:160)
+  abs((a+da)/a_next-1) >= 1e-6        real(4) (:55)
 """
 import math
 
@@ -14,27 +22,35 @@ import numpy as np
 f32 = np.float32
 
 
+def _derivs(a_x, om, ol, w, omhsq):
+    """adot, addot, atdot at a_x (real(8)); om, ol, w real(4) parameters."""
+    e3 = float(f32(-3) * w)                                   # -3*wde, real(4)
+    c1 = float(f32(f32(f32(1) - om) - ol))                    # 1.0-omega_m-omega_l
+    c2 = float(f32(1.5) * f32(f32(1) - w))                    # 1.5*(1.0-wde)
+    c3 = float(f32(f32(1.5) * f32(f32(2) - f32(3) * w)) * f32(f32(1) - w))
+    a3rlm = (math.pow(a_x, e3) * float(ol)) / float(om)
+    arkm = (a_x * c1) / float(om)
+    adot = math.sqrt((omhsq * ((a_x * a_x) * a_x)) * ((1.0 + arkm) + a3rlm))
+    addot = ((a_x * a_x) * omhsq) * ((1.5 + 2.0 * arkm) + c2 * a3rlm)
+    atdot = ((a_x * adot) * omhsq) * ((3.0 + 6.0 * arkm) + c3 * a3rlm)
+    return adot, addot, atdot
+
+
 def expansion(a0, dt0, omega_m, omega_l, wde=-1.0):
-    """da1, da2 of one step: W/timestep.f90:133-177 (the default branch,
-    without the radiation term).  a0, dt0 real(4); returns real(4) pair."""
+    """da1, da2 of one step: W/timestep.f90:137-177 (the default branch,
+    without the radiation term).  a0, dt0 real(4); returns a real(4) pair."""
     om, ol, w = f32(omega_m), f32(omega_l), f32(wde)
+    a0 = f32(a0)
     dt_x = f32(f32(dt0) / f32(2))
-    omHsq = 4.0 / 9.0
-
-    def derivs(a_x):
-        a3rlm = a_x ** float(f32(-3) * w) * float(ol) / float(om)
-        arkm = a_x * (1.0 - float(om) - float(ol)) / float(om)
-        adot = math.sqrt(omHsq * a_x ** 3 * (1.0 + arkm + a3rlm))
-        addot = a_x ** 2 * omHsq * (1.5 + 2.0 * arkm + 1.5 * (1.0 - float(w)) * a3rlm)
-        atdot = a_x * adot * omHsq * (3.0 + 6.0 * arkm + 1.5 * (2.0 - 3.0 * float(w)) * (1.0 - float(w)) * a3rlm)
-        return adot, addot, atdot
-
-    d = float(dt_x)
-    adot, addot, atdot = derivs(float(f32(a0)))
-    da1 = f32(adot * d + (addot * d ** 2) / 2.0 + (atdot * d ** 3) / 6.0)
-    a_x = float(f32(f32(a0) + da1))
-    adot, addot, atdot = derivs(a_x)
-    da2 = f32(adot * d + (addot * d ** 2) / 2.0 + (atdot * d ** 3) / 6.0)
+    dt2 = f32(dt_x * dt_x)
+    dt3 = f32(dt2 * dt_x)
+    omhsq = float(f32(4.0) / f32(9.0))
+    d, d2, d3 = float(dt_x), float(dt2), float(dt3)
+    adot, addot, atdot = _derivs(float(a0), om, ol, w, omhsq)
+    da1 = f32((adot * d + (addot * d2) / 2.0) + (atdot * d3) / 6.0)
+    a_x = float(f32(a0 + da1))
+    adot, addot, atdot = _derivs(a_x, om, ol, w, omhsq)
+    da2 = f32((adot * d + (addot * d2) / 2.0) + (atdot * d3) / 6.0)
     return da1, da2
 
 
@@ -87,7 +103,7 @@ class Clock:
             if self.cur_checkpoint == len(self.z_checkpoint):
                 self.final_step = True
             n = 0
-            while abs(float(f32(a + da) / a_next) - 1) >= 1e-6 and n < 100:
+            while abs(f32(f32(f32(a + da) / a_next) - f32(1))) >= f32(1e-6) and n < 100:
                 dt = f32(dt * f32(a_next - a) / da)
                 da1, da2 = expansion(a, dt, self.om, self.ol)
                 da = f32(da1 + da2)

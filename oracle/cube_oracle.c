@@ -1001,3 +1001,48 @@ int64_t *oc_spine_ptr(ocube *c, int which) {
   int64_t *a[8] = {c->idx_b_l, c->idx_b_r, c->ppl0, c->pplr, c->pprl, c->ppr0, c->ppl, c->ppr};
   return a[which];
 }
+
+/* ---- host time integration ---------------------------------------------- */
+/* expansion of W/timestep.f90:137-177 (the branch built without
+   -Dnew_expansion), kind for kind as gfortran -O3 evaluates it on x86-64:
+     dt_x=dt0/2                      real(4)
+     omHsq=4.0/9.0                   a real(4) quotient widened to real(8)
+     a_x**(-3*wde)                   real exponent: a libm pow call (gcc
+                                     expands pow(x,c) inline only for
+                                     c in {-1,0,1,2} without -ffast-math)
+     a_x**3, a_x**2                  integer powers: (x*x)*x and x*x
+     dt_x**2, dt_x**3                real(4) products
+     omega_m, omega_l, wde           real(4) parameters (W/parameters.f90:101-107)
+   Mixed real(8)*real(4) operands promote where Fortran promotes them. */
+static void oc_derivs(double a_x, float om, float ol, float wde, double omhsq, double *adot,
+                      double *addot, double *atdot) {
+  const float e3 = -3.0f * wde;                       /* -3*wde */
+  const float c1 = (1.0f - om) - ol;                  /* 1.0-omega_m-omega_l */
+  const float c2 = 1.5f * (1.0f - wde);               /* 1.5*(1.0-wde) */
+  const float c3 = (1.5f * (2.0f - 3.0f * wde)) * (1.0f - wde);
+  const double a3rlm = (pow(a_x, (double)e3) * (double)ol) / (double)om;
+  const double arkm = (a_x * (double)c1) / (double)om;
+  const double ax3 = (a_x * a_x) * a_x;
+  *adot = sqrt((omhsq * ax3) * ((1.0 + arkm) + a3rlm));
+  *addot = ((a_x * a_x) * omhsq) * ((1.5 + 2.0 * arkm) + (double)c2 * a3rlm);
+  *atdot = ((a_x * *adot) * omhsq) * ((3.0 + 6.0 * arkm) + (double)c3 * a3rlm);
+}
+
+void oc_expansion(float a0, float dt0, float om, float ol, float wde, float *da1, float *da2) {
+  const float dt_x = dt0 / 2.0f;
+  const float dt2 = dt_x * dt_x;
+  const float dt3 = dt2 * dt_x;
+  const double omhsq = (double)(4.0f / 9.0f);
+  double adot, addot, atdot;
+  oc_derivs((double)a0, om, ol, wde, omhsq, &adot, &addot, &atdot);
+  *da1 = (float)((adot * (double)dt_x + (addot * (double)dt2) / 2.0) + (atdot * (double)dt3) / 6.0);
+  const float a1 = a0 + *da1;                         /* a_x=a0+da1, real(4) sum */
+  oc_derivs((double)a1, om, ol, wde, omhsq, &adot, &addot, &atdot);
+  *da2 = (float)((adot * (double)dt_x + (addot * (double)dt2) / 2.0) + (atdot * (double)dt3) / 6.0);
+}
+
+/* Vectorised form for the tests: n (a0, dt0) pairs. */
+void oc_expansion_n(int64_t n, const float *a0, const float *dt0, float om, float ol, float wde,
+                    float *da1, float *da2) {
+  for (int64_t i = 0; i < n; i++) oc_expansion(a0[i], dt0[i], om, ol, wde, &da1[i], &da2[i]);
+}

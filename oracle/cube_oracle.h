@@ -110,6 +110,11 @@ float oc_tanf_code(int64_t izipv, int32_t code);
    the count sort of WU/initial_conditions.f90:684-786 (plain cell order) */
 int oc_state_from_xv(ocube *c, int64_t n, const double *x, const double *v,
                      const int32_t *pid, float sigma_vi);
+/* expansion of W/timestep.f90:137-177 with the reference's real(4)/real(8)
+   kinds: da1, da2 of one step from a0 over dt0 (host time integration) */
+void oc_expansion(float a0, float dt0, float om, float ol, float wde, float *da1, float *da2);
+void oc_expansion_n(int64_t n, const float *a0, const float *dt0, float om, float ol, float wde,
+                    float *da1, float *da2);
 ocube *oc_alloc(void);
 void oc_free(ocube *c);
 void oc_info(const ocube *c, int64_t *out16);

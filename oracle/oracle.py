@@ -74,6 +74,7 @@ def lib():
             getattr(L, name).restype = P
         L.oc_spine_ptr.argtypes = [P, ctypes.c_int]
         L.oc_spine_ptr.restype = P
+        L.oc_expansion_n.argtypes = [i64, P, P, f32, f32, f32, P, P]
         _LIB = L
     return _LIB
 
@@ -414,3 +415,14 @@ def encode_x_drift(izipx, code, dtv):
 
 def tanf_code(izipv, code):
     return lib().oc_tanf_code(izipv, int(code))
+
+
+def expansion(a0, dt0, omega_m, omega_l, wde=-1.0):
+    """W/timestep.f90:137-177 with the reference's kinds (oc_expansion):
+    arrays of real(4) a0, dt0 -> real(4) da1, da2."""
+    a0 = np.ascontiguousarray(np.atleast_1d(a0), np.float32)
+    dt0 = np.ascontiguousarray(np.broadcast_to(np.asarray(dt0, np.float32), a0.shape))
+    da1 = np.zeros_like(a0)
+    da2 = np.zeros_like(a0)
+    lib().oc_expansion_n(a0.size, _p(a0), _p(dt0), omega_m, omega_l, wde, _p(da1), _p(da2))
+    return da1, da2
