@@ -99,6 +99,9 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
     return CUBE_ERR_INVALID;
   const int64_t nt = p->nc / p->nnt;
   if (nt < p->ncb) return CUBE_ERR_INVALID;
+  /* the drift's within-cell order key keeps k_src-k_dst+2*ncb in 5 bits,
+     and the coarse and fine meshes (nc, 4*nc) need built FFT sizes */
+  if (p->ncb > 7 || !cube_fft_supported(p->nc) || !cube_fft_supported(4 * p->nc)) return CUBE_ERR_INVALID;
   cube_ctx *c = new (std::nothrow) cube_ctx;
   if (!c) return CUBE_ERR_ALLOC;
   memset(c, 0, sizeof(*c));

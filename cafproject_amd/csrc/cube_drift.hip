@@ -33,7 +33,7 @@
 #define DT 256
 static const double WEIGHT_V = (double)0.1f; /* real(8),parameter :: weight_v=0.1 */
 
-enum { DERR_RANGE = 1, DERR_KEY = 2, DERR_BIG = 4, DERR_XOVER = 8 };
+enum { DERR_RANGE = 1, DERR_KEY = 2, DERR_XOVER = 8 };
 
 /* ---------------- small utilities ---------------- */
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -404,23 +404,8 @@ __global__ void k_sort_small(const int32_t *__restrict__ cnt, const int64_t *__r
     if (q < n) s[q] = v[q];
 }
 
-/* one workgroup per big cell: bitonic sort in LDS (n <= BIG_SORT) */
-__global__ __launch_bounds__(1024) void k_sort_big(const int32_t *__restrict__ cnt,
-                                                   const int64_t *__restrict__ start,
-                                                   const int32_t *__restrict__ biglist, uint64_t *__restrict__ a,
-                                                   unsigned long long *__restrict__ counters) {
-  extern __shared__ uint64_t sk[];
-  const int32_t cidx = biglist[blockIdx.x];
-  const int n = cnt[cidx];
-  if (n > BIG_SORT) {
-    if (threadIdx.x == 0) atomicOr(counters + 2, (unsigned long long)DERR_BIG);
-    return;
-  }
-  int m = 1;
-  while (m < n) m <<= 1;
-  uint64_t *s = a + start[cidx];
-  for (int q = threadIdx.x; q < m; q += blockDim.x) sk[q] = q < n ? s[q] : ~0ull;
-  __syncthreads();
+/* bitonic sort of sk[0..m) in LDS, m a power of two */
+__device__ static void lds_bitonic(uint64_t *sk, int m) {
   for (int size = 2; size <= m; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       for (int q = threadIdx.x; q < m; q += blockDim.x) {
@@ -436,67 +421,71 @@ __global__ __launch_bounds__(1024) void k_sort_big(const int32_t *__restrict__ c
       }
       __syncthreads();
     }
-  for (int q = threadIdx.x; q < n; q += blockDim.x) s[q] = sk[q];
+}
+/* number of elements of the ascending run r[0..n) below x */
+__device__ static int lower_bound_u64(const uint64_t *r, int n, uint64_t x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (r[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+/* One workgroup per big cell.  Up to BIG_SORT keys: a bitonic sort in LDS.
+   Larger cells (a collapsed halo core can hold more particles than LDS
+   takes): BIG_SORT-key runs sorted in LDS, then pairwise merges of runs in
+   global memory between the cell's segment and the same range of tmp (the
+   placement keys, dead by now).  Keys within a cell are distinct (the low
+   bits are the particle index), so an element's merged position is its
+   index in its run plus its rank in the other run. */
+__global__ __launch_bounds__(1024) void k_sort_big(const int32_t *__restrict__ cnt,
+                                                   const int64_t *__restrict__ start,
+                                                   const int32_t *__restrict__ biglist, uint64_t *__restrict__ a,
+                                                   uint64_t *__restrict__ tmp) {
+  extern __shared__ uint64_t sk[];
+  const int32_t cidx = biglist[blockIdx.x];
+  const int n = cnt[cidx];
+  uint64_t *s = a + start[cidx];
+  if (n <= BIG_SORT) {
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int q = threadIdx.x; q < m; q += blockDim.x) sk[q] = q < n ? s[q] : ~0ull;
+    __syncthreads();
+    lds_bitonic(sk, m);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s[q] = sk[q];
+    return;
+  }
+  for (int c0 = 0; c0 < n; c0 += BIG_SORT) {
+    const int len = min(BIG_SORT, n - c0);
+    for (int q = threadIdx.x; q < BIG_SORT; q += blockDim.x) sk[q] = q < len ? s[c0 + q] : ~0ull;
+    __syncthreads();
+    lds_bitonic(sk, BIG_SORT);
+    for (int q = threadIdx.x; q < len; q += blockDim.x) s[c0 + q] = sk[q];
+    __syncthreads();
+  }
+  uint64_t *src = s, *dst = tmp + start[cidx];
+  for (int w = BIG_SORT; w < n; w <<= 1) {
+    for (int lo = 0; lo < n; lo += 2 * w) {
+      const int mid = min(lo + w, n), hi = min(lo + 2 * w, n);
+      for (int q = lo + (int)threadIdx.x; q < hi; q += blockDim.x) {
+        const uint64_t x = src[q];
+        const int pos = q < mid ? (q - lo) + lower_bound_u64(src + mid, hi - mid, x)
+                                : (q - mid) + lower_bound_u64(src + lo, mid - lo, x);
+        dst[lo + pos] = x;
+      }
+    }
+    __syncthreads();
+    uint64_t *t = src;
+    src = dst;
+    dst = t;
+  }
+  if (src != s) {
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s[q] = src[q];
+  }
 }
 
 /* ---------------- per-cell finalize ---------------- */
-template <int IX, int IV>
-__global__ void k_drift_final(const typename IntOf<IX>::T *__restrict__ xp, const typename IntOf<IV>::T *__restrict__ vp,
-                              const int32_t *__restrict__ pid, const float *__restrict__ vfield_old,
-                              const int32_t *__restrict__ cnt, const int64_t *__restrict__ start,
-                              const uint64_t *__restrict__ keys, const double *__restrict__ vdec, DriftP P, double venc,
-                              typename IntOf<IX>::T *__restrict__ xp2, typename IntOf<IV>::T *__restrict__ vp2,
-                              int32_t *__restrict__ pid2, float *__restrict__ vfield_new, int64_t ncells) {
-  const int64_t cidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (cidx >= ncells) return;
-  const Geo g = P.g;
-  const int nt = g.nt, ncb = g.ncb;
-  const int i = (int)(cidx % nt), j = (int)((cidx / nt) % nt), k = (int)((cidx / ((int64_t)nt * nt)) % nt);
-  const int t = (int)(cidx / ((int64_t)nt * nt * nt));
-  const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
-  const int n = cnt[cidx];
-  const int64_t s0 = start[cidx];
-  float acc[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) acc[a] = (float)((double)vfield_old[3 * cidx + a] * WEIGHT_V);
-  for (int q = 0; q < n; ++q) {
-    const uint64_t kk = keys[s0 + q];
-    const int64_t p = (int64_t)(kk & KEY_PMASK);
-    const int rx = (int)((kk >> 40) & 31), ry = (int)((kk >> 45) & 31), rz = (int)((kk >> 50) & 31);
-    /* source cell = destination + (r - 2ncb), in the destination tile frame */
-    const int64_t sc = cid_global(g, tx * nt + i + rx - 2 * ncb, ty * nt + j + ry - 2 * ncb,
-                                  tz * nt + k + rz - 2 * ncb);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double vreal = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vfield_old[3 * sc + a];
-      acc[a] = (float)((double)acc[a] + vreal);
-    }
-  }
-  float vn[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    vn[a] = (float)((double)acc[a] / ((double)n + WEIGHT_V));
-    vfield_new[3 * cidx + a] = vn[a];
-  }
-  for (int q = 0; q < n; ++q) {
-    const uint64_t kk = keys[s0 + q];
-    const int64_t p = (int64_t)(kk & KEY_PMASK);
-    const int rx = (int)((kk >> 40) & 31), ry = (int)((kk >> 45) & 31), rz = (int)((kk >> 50) & 31);
-    const int64_t sc = cid_global(g, tx * nt + i + rx - 2 * ncb, ty * nt + j + ry - 2 * ncb,
-                                  tz * nt + k + rz - 2 * ncb);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double vreal = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vfield_old[3 * sc + a];
-      /* xp_new=xp+nint(dt_mid*vreal/(x_resolution*ncell)) (:115) */
-      const double qx = ((double)P.dt_mid * vreal) / (P.xres * 4.0);
-      xp2[3 * (s0 + q) + a] = (typename IntOf<IX>::T)cube_wrap(IX, (int64_t)xp[3 * p + a] + (int64_t)cube_nint(qx));
-      /* vp_new=nint(...atan(...*(vreal-vfield_new))...) (:116-117) */
-      vp2[3 * (s0 + q) + a] = (typename IntOf<IV>::T)cube_encode_v(IV, venc, vreal - (double)vn[a]);
-    }
-    if (pid) pid2[s0 + q] = pid[p];
-  }
-}
-
 /*
  * Finalize by slot ranges: a workgroup owns FB_CELLS consecutive new cells
  * and the contiguous slot range of their particles.  Lanes gather the
@@ -772,7 +761,8 @@ __global__ void k_place_keys(const int32_t *dest, const uint64_t *key, int64_t n
   }
 }
 /* sort every cell's key segment ascending (small ones in registers, big
-   ones with a workgroup bitonic sort) */
+   ones with a workgroup bitonic sort, huge ones by merging sorted runs);
+   c->key, dead once the keys are placed, is the merge buffer */
 extern "C" int cube_sort_small_segments(cube_ctx *c, const int32_t *cnt, const int64_t *start, uint64_t *keys) {
   unsigned long long *ctr = (unsigned long long *)c->ured;
   int st = fill_i64(c, (int64_t *)ctr + 3, 1, 0);
@@ -786,10 +776,8 @@ extern "C" int cube_sort_small_segments(cube_ctx *c, const int32_t *cnt, const i
   CUBE_CHECK(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
   if (hc[3]) {
     hipLaunchKernelGGL(k_sort_big, dim3((unsigned)hc[3]), dim3(1024), BIG_SORT * sizeof(uint64_t), c->stream, cnt,
-                       start, biglist, keys, ctr);
+                       start, biglist, keys, c->key);
     CUBE_LAUNCH_CHECK();
-    CUBE_CHECK(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
-    if (hc[2] & DERR_BIG) return CUBE_ERR_INVALID;
   }
   return CUBE_OK;
 }
@@ -836,10 +824,8 @@ template <int IX, int IV> static int drift_impl(cube_ctx *c, DriftP P, double ve
   CUBE_CHECK(hipMemcpy(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   if (hc[3]) {
     hipLaunchKernelGGL(k_sort_big, dim3((unsigned)hc[3]), dim3(1024), BIG_SORT * sizeof(uint64_t), c->stream,
-                       c->rhoc2, c->cstart2, biglist, c->sortbuf, ctr);
+                       c->rhoc2, c->cstart2, biglist, c->sortbuf, c->key); /* key is dead after placement */
     CUBE_LAUNCH_CHECK();
-    CUBE_CHECK(hipMemcpy(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    if (hc[2] & DERR_BIG) return CUBE_ERR_INVALID;
   }
   hipLaunchKernelGGL((k_drift_final2<IX, IV>), dim3((unsigned)nfb), dim3(256), 0, c->stream, (const TX *)c->xp,
                      (const TV *)c->vp, c->p.has_pid ? c->pid : nullptr, c->vfield, c->cstart2, c->sortbuf,

@@ -783,8 +783,22 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x1(const float2 *__restrict__ spe
 }
 
 /* ---------------- host side ---------------- */
+#define FFT_SIZES(X) \
+  X(12) X(16) X(18) X(20) X(24) X(32) X(48) X(64) X(72) X(80) X(96) X(128) X(192) X(256) X(512) X(1024) X(2048)
+
+int cube_fft_supported(int64_t n) {
+  switch (n) {
+#define CASE(N) case N:
+    FFT_SIZES(CASE)
+#undef CASE
+    return 1;
+  }
+  return 0;
+}
+
 int cube_fft_create(cube_ctx *c, int n, FFTPlan **out) {
   (void)c;
+  if (!cube_fft_supported(n)) return CUBE_ERR_INVALID;
   FFTPlan *p = new FFTPlan;
   p->n = n;
   float2 *h = new float2[n];
@@ -878,8 +892,6 @@ template <int N> struct Launch {
     return CUBE_OK;
   }
 };
-
-#define FFT_SIZES(X) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512) X(1024) X(2048)
 
 int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out) {
   int st = CUBE_ERR_INVALID;

@@ -279,6 +279,8 @@ int cube_vdec_table(cube_ctx *c, double *dst, float sigma);
 int cube_scan_counts(cube_ctx *c, const int32_t *cnt, int64_t *start, int64_t n, int64_t *total);
 int cube_tile_boxsum(cube_ctx *c, const int32_t *rhoc, int w, int64_t *host_out);
 
+/* 1 when the line-FFT kernels are built for size n (12..2048, factors 2, 3) */
+int cube_fft_supported(int64_t n);
 int cube_fft_create(cube_ctx *c, int n, FFTPlan **out);
 void cube_fft_destroy(FFTPlan *p);
 int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);

@@ -44,69 +44,6 @@ __device__ static inline void order_z(Src &s, int k0) {
   }
 }
 
-/* ---------------- fine CIC deposit, W/pm.f90:50-108 ---------------- */
-template <int IX>
-__global__ __launch_bounds__(MT) void k_fine_deposit(const typename IntOf<IX>::T *__restrict__ xp,
-                                                     const int32_t *__restrict__ rhoc,
-                                                     const int64_t *__restrict__ cstart, Geo g, float mass,
-                                                     float *__restrict__ rho, double *__restrict__ part) {
-  const int nf = 4 * g.nc, nft = 4 * g.nt;
-  const int64_t nf3 = (int64_t)nf * nf * nf;
-  double tsum = 0;
-  for (int64_t f = blockIdx.x * (int64_t)MT + threadIdx.x; f < nf3; f += (int64_t)gridDim.x * MT) {
-    const int fx = (int)(f % nf), fy = (int)((f / nf) % nf), fz = (int)(f / ((int64_t)nf * nf));
-    const int fg[3] = {fx, fy, fz};
-    int T[3], lf[3];
-    Src s[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      T[a] = fg[a] / nft;
-      lf[a] = fg[a] - T[a] * nft + 1;            /* 1-based local fine index */
-      const int C = (lf[a] - 1) / 4 + 1, sub = (lf[a] - 1) % 4;
-      if (sub == 0) {
-        s[a].n = 2; s[a].c[0] = C - 1; s[a].c[1] = C;
-      } else {
-        s[a].n = 1; s[a].c[0] = C;
-      }
-    }
-    order_z(s[2], 2 - g.ncb);
-    float acc = 0.f;
-    for (int qz = 0; qz < s[2].n; ++qz)
-      for (int qy = 0; qy < s[1].n; ++qy)
-        for (int qx = 0; qx < s[0].n; ++qx) {
-          const int ci[3] = {s[0].c[qx], s[1].c[qy], s[2].c[qz]};
-          const int64_t cc = cid_global(g, T[0] * g.nt + ci[0] - 1, T[1] * g.nt + ci[1] - 1, T[2] * g.nt + ci[2] - 1);
-          const int np = rhoc[cc];
-          const int64_t p0 = cstart[cc];
-          for (int l = 0; l < np; ++l) {
-            float w[3];
-            bool hit = true;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const int32_t code = (int32_t)xp[3 * (p0 + l) + a];
-              const float tempx = (float)((double)(4 * (ci[a] - 1)) + 4.0 * cube_xoff(IX, code));
-              const int i1 = (int)floorf(tempx) + 1;
-              const float d1 = (float)i1 - tempx;
-              if (i1 == lf[a]) w[a] = d1;
-              else if (i1 + 1 == lf[a]) w[a] = 1.0f - d1;
-              else hit = false;
-            }
-            if (hit) acc = acc + w[0] * w[1] * w[2] * mass;
-          }
-        }
-    rho[f] = acc;
-    tsum += (double)acc;
-  }
-  __shared__ double red[MT];
-  red[threadIdx.x] = tsum;
-  __syncthreads();
-  for (int o = MT / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
 /* the coarse CIC coordinate ((cell-1)+(x offset))-0.5 of W/pm.f90:213,335
    rounded to real(4); c = cell-1 (tile frame or global).  With 1-byte codes
    c + (u+0.5)/256 has at most 24 significant bits for c < 2^15, so the

@@ -109,3 +109,17 @@ end program
                         "-lcube_hip", "-Wl,-rpath," + libdir], check=True, capture_output=True)
         got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     assert got == [ctypes.sizeof(cube.Params), ctypes.sizeof(cube.VStats), ctypes.sizeof(cube.PMOut)]
+
+
+@pytest.mark.parametrize("kw", [dict(ncb=8), dict(nc=40, nnt=2), dict(nc=16, nnt=3), dict(nc=24, nnt=8)])
+def test_create_rejects_unsupported_sizes(kw):
+    """cube_ctx_create refuses, before touching a device, what it cannot
+    run: ncb > 7 (the drift's order key keeps k_src-k_dst+2 ncb in 5 bits),
+    coarse or fine meshes without a built FFT size (nc=40: 40 and 160),
+    nc not divisible by nnt, and tiles smaller than the buffer."""
+    from cafproject_amd import cube
+    args = dict(nc=24, nnt=2)
+    args.update(kw)
+    with pytest.raises(cube.CubeError) as e:
+        cube.Cube(**args)
+    assert e.value.code == 12
