@@ -14,6 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
+#include <omp.h>
 
 /* ---- constants (W/parameters.f90) ------------------------------------- */
 #define PI_F 3.14159274101257324f            /* pi=4*atan(1.), real(4) */
@@ -414,6 +415,29 @@ void oc_buffer_xvp(ocube *c, int dox, int dov) {
 }
 
 /* ---- update_xp, W/update_particle.f90:16-256 ----------------------------- */
+/* 1 when no particle of tile t moves more than (nlayer-1)/2 cells in z
+   during the drift, the premise of the reference's layered parallel loops */
+static int drift_layers_disjoint(ocube *c, int64_t t, float dt_mid, double vc, int64_t nlayer) {
+  const int64_t nt = c->nt, ncb = c->ncb, lb = LB(c), ncell = c->ncell, h = (nlayer - 1) / 2;
+  int ok = 1;
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : ok)
+  for (int64_t k = lb; k <= nt + ncb; k++)
+    for (int64_t j = lb; j <= nt + ncb; j++)
+      for (int64_t i = lb; i <= nt + ncb; i++) {
+        const int64_t np = RHOC(i, j, k, t);
+        const int64_t nzero = c->idx_b_r[brow(c, j, k, t)] - row_suffix(c, i, j, k, t);
+        for (int64_t l = 1; l <= np; l++) {
+          const int64_t ip = nzero + l;
+          const double xq = ((double)k - 1.0) + oc_decode_x(c->izipx, XP(2, ip));
+          double vr = (double)oc_tanf_code(c->izipv, VP(2, ip)) / vc;
+          vr = vr + (double)VF(2, i, j, k, t);
+          const int64_t ig = (int64_t)ceil(xq + ((double)dt_mid * vr) / (double)ncell);
+          if (ig - k > h || k - ig > h) ok = 0;
+        }
+      }
+  return ok;
+}
+
 int oc_update_xp(ocube *c, float dt_old, float dt, float vz_max, float sigma_vi,
                  double *stats, float *sigma_vi_new) {
   const int64_t nt = c->nt, ncb = c->ncb, nnt = c->nnt, lb = LB(c), ncell = c->ncell;
@@ -451,8 +475,17 @@ int oc_update_xp(ocube *c, float dt_old, float dt, float vz_max, float sigma_vi,
         memset(xp_new, 0, 3 * c->np_tile_max * sizeof(int16_t));
         memset(vp_new, 0, 3 * c->np_tile_max * sizeof(int16_t));
         if (pid_new) memset(pid_new, 0, c->np_tile_max * sizeof(int32_t));
+        /* The reference runs each layer's k slabs in parallel (OpenMP,
+           W/update_particle.f90:55-58): slabs nlayer apart reach disjoint
+           destination planes when no particle moves more than
+           (nlayer-1)/2 cells in z, so every destination cell still sums in
+           (ilayer,k,j,i,l) order.  The oracle checks that premise per tile
+           and runs the slabs serially where it fails. */
+        const int par = drift_layers_disjoint(c, t, dt_mid, vc, nlayer);
         /* pass 1: count and vfield, layered (ilayer,k,j,i,l) order (:55-78) */
-        for (int64_t ilayer = 0; ilayer < nlayer && status == OC_OK; ilayer++)
+        for (int64_t ilayer = 0; ilayer < nlayer && status == OC_OK; ilayer++) {
+          int st1 = OC_OK;
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : st1) if (par)
           for (int64_t k = lb + ilayer; k <= nt + ncb; k += nlayer)
             for (int64_t j = lb; j <= nt + ncb; j++)
               for (int64_t i = lb; i <= nt + ncb; i++) {
@@ -463,21 +496,27 @@ int oc_update_xp(ocube *c, float dt_old, float dt, float vz_max, float sigma_vi,
                   const int64_t ip = nzero + l;
                   int64_t ig[3];
                   double vreal[3];
+                  int bad = 0;
                   for (int d = 0; d < 3; d++) {
                     const double xq = ((double)ijk[d] - 1.0) + oc_decode_x(c->izipx, XP(d, ip));
                     vreal[d] = (double)oc_tanf_code(c->izipv, VP(d, ip)) / vc;
                     vreal[d] = vreal[d] + (double)VF(d, i, j, k, t);
                     const double deltax = ((double)dt_mid * vreal[d]) / (double)ncell;
                     ig[d] = (int64_t)ceil(xq + deltax);
-                    if (ig[d] < lb2 || ig[d] > nt + 2 * ncb) status = OC_ERR_DISPLACEMENT;
+                    if (ig[d] < lb2 || ig[d] > nt + 2 * ncb) bad = 1;
                   }
-                  if (status != OC_OK) break;
+                  if (bad) {
+                    st1 = OC_ERR_DISPLACEMENT;
+                    continue;
+                  }
                   const int64_t e = E(ig[0], ig[1], ig[2]);
                   rhoce[e] += 1;
                   for (int d = 0; d < 3; d++)
                     vfn[3 * e + d] = (float)((double)vfn[3 * e + d] + vreal[d]);
                 }
               }
+          if (st1 != OC_OK) status = st1;
+        }
         if (status != OC_OK) break;
         /* vfield_new/(rhoce+weight_v) (:81-83) */
         for (int64_t q = 0; q < ne * ne * ne; q++)
@@ -515,8 +554,9 @@ int oc_update_xp(ocube *c, float dt_old, float dt, float vz_max, float sigma_vi,
           if (ov > c->overhead_tile) c->overhead_tile = ov;
         }
         if (ntile_tot > c->np_tile_max) { status = OC_ERR_TILE_OVERFLOW; break; }
-        /* pass 2: scatter with re-quantisation (:97-126) */
+        /* pass 2: scatter with re-quantisation (:97-126), layered as pass 1 */
         for (int64_t ilayer = 0; ilayer < nlayer; ilayer++)
+#pragma omp parallel for schedule(dynamic, 1) if (par)
           for (int64_t k = lb + ilayer; k <= nt + ncb; k += nlayer)
             for (int64_t j = lb; j <= nt + ncb; j++)
               for (int64_t i = lb; i <= nt + ncb; i++) {
@@ -617,7 +657,11 @@ double oc_fine_deposit(const ocube *c_in, int64_t itx, int64_t ity, int64_t itz,
   const int64_t sx = nfe + 2;
 #define RF(x, y, z) rho_f[((x) - 1) + sx * (((y) - 1) + nfe * ((z) - 1))]
   memset(rho_f, 0, sx * nfe * nfe * sizeof(float));
+  /* layers of k slabs 3 apart touch disjoint planes of rho_f, so the slabs
+     of a layer run in parallel as in the reference (W/pm.f90:52-55) and
+     every cell sums in (ilayer,k,j,i,l) order */
   for (int64_t ilayer = 0; ilayer < 3; ilayer++)
+#pragma omp parallel for schedule(dynamic, 1)
     for (int64_t k = 2 - ncb + ilayer; k <= nt + ncb - 1; k += 3)
       for (int64_t j = 2 - ncb; j <= nt + ncb - 1; j++)
         for (int64_t i = 2 - ncb; i <= nt + ncb - 1; i++) {
@@ -662,6 +706,7 @@ void oc_fine_kick(ocube *c, int64_t itx, int64_t ity, int64_t itz, const float *
   const int64_t t = tile_id(c, itx, ity, itz);
   const double vc = vconst(sigma_vi);
 #define FF(d, x, y, z) force_f[(d) + 3 * (((x) - nfb) + nf2 * (((y) - nfb) + nf2 * ((z) - nfb)))]
+#pragma omp parallel for schedule(dynamic, 1)
   for (int64_t k = 1; k <= nt; k++)
     for (int64_t j = 1; j <= nt; j++)
       for (int64_t i = 1; i <= nt; i++) {
@@ -714,6 +759,7 @@ double oc_coarse_deposit(const ocube *c_in, float mass_p, float *r3) {
         const int64_t t = tile_id(c, tx, ty, tz);
         memset(r3t, 0, m * m * m * sizeof(float));
         for (int64_t ilayer = 0; ilayer < 3; ilayer++)
+#pragma omp parallel for schedule(dynamic, 1)
           for (int64_t k = ilayer; k <= nt + 1; k += 3)
             for (int64_t j = 0; j <= nt + 1; j++)
               for (int64_t i = 0; i <= nt + 1; i++) {
@@ -765,6 +811,7 @@ void oc_coarse_kick(ocube *c, const float *force_c, float a_mid, float dt,
       for (int64_t tx = 1; tx <= nnt; tx++) {
         const int64_t t = tile_id(c, tx, ty, tz);
         const int64_t txyz[3] = {tx, ty, tz};
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : vm[:3])
         for (int64_t k = 1; k <= nt; k++)
           for (int64_t j = 1; j <= nt; j++)
             for (int64_t i = 1; i <= nt; i++) {
@@ -856,13 +903,19 @@ int oc_pp_force(ocube *c, float a_mid, float dt, float sigma_vi, float mass_p,
                 HOC(iv[0], iv[1], iv[2]) = ipll1;
               }
             }
+        /* particles only read positions of others and write their own
+           velocity, so the fine cells run in parallel; the running max of
+           f2 in real(4) (:123) is order-free (rounding is monotone) */
+        int64_t n1 = 0, npr = 0;
+        double fmx = (double)(float)c->pp_ftemp;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n1, npr) reduction(max : fmx)
         for (int64_t igz = 1; igz <= nft; igz++)
           for (int64_t igy = 1; igy <= nft; igy++)
             for (int64_t igx = 1; igx <= nft; igx++) {
               int32_t ipll1 = HOC(igx, igy, igz);
               const int64_t g1[3] = {igx, igy, igz};
               while (ipll1 != 0) {
-                c->itest1++;
+                n1++;
                 const int64_t ip1 = ipll1 + ip_offset;
                 double x1[3], vreal[3], f_tot[3] = {0, 0, 0};
                 for (int d = 0; d < 3; d++) {
@@ -876,7 +929,7 @@ int oc_pp_force(ocube *c, float a_mid, float dt, float sigma_vi, float mass_p,
                       int32_t ipll2 = HOC(ii, jj, kk);
                       const int64_t g2[3] = {ii, jj, kk};
                       while (ipll2 != 0) {
-                        c->npairs++;
+                        npr++;
                         const int64_t ip2 = ipll2 + ip_offset;
                         double x21[3];
                         for (int d = 0; d < 3; d++) {
@@ -902,11 +955,14 @@ int oc_pp_force(ocube *c, float a_mid, float dt, float sigma_vi, float mass_p,
                   VP(d, ip1) = (int16_t)oc_encode_v(c->izipv, vreal[d], sigma_vi);
                 }
                 const double f2 = (f_tot[0] * f_tot[0] + f_tot[1] * f_tot[1]) + f_tot[2] * f_tot[2];
-                const double m = (double)(float)c->pp_ftemp;
-                c->pp_ftemp = (double)(float)(f2 > m ? f2 : m);
+                const double f2r = (double)(float)f2;
+                if (f2r > fmx) fmx = f2r;
                 ipll1 = ll[ipll1 - 1];
               }
             }
+        c->itest1 += n1;
+        c->npairs += npr;
+        c->pp_ftemp = fmx;
       }
 #undef HOC
   free(hoc); free(ll);
@@ -1046,3 +1102,7 @@ void oc_expansion_n(int64_t n, const float *a0, const float *dt0, float om, floa
                     float *da1, float *da2) {
   for (int64_t i = 0; i < n; i++) oc_expansion(a0[i], dt0[i], om, ol, wde, &da1[i], &da2[i]);
 }
+
+/* host threads of the oracle's parallel loops (OpenMP) */
+void oc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oc_get_threads(void) { return omp_get_max_threads(); }

@@ -115,6 +115,9 @@ int oc_state_from_xv(ocube *c, int64_t n, const double *x, const double *v,
 void oc_expansion(float a0, float dt0, float om, float ol, float wde, float *da1, float *da2);
 void oc_expansion_n(int64_t n, const float *a0, const float *dt0, float om, float ol, float wde,
                     float *da1, float *da2);
+/* OpenMP threads of the parallel loops; results do not depend on it */
+void oc_set_threads(int n);
+int oc_get_threads(void);
 ocube *oc_alloc(void);
 void oc_free(ocube *c);
 void oc_info(const ocube *c, int64_t *out16);

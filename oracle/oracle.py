@@ -75,6 +75,8 @@ def lib():
         L.oc_spine_ptr.argtypes = [P, ctypes.c_int]
         L.oc_spine_ptr.restype = P
         L.oc_expansion_n.argtypes = [i64, P, P, f32, f32, f32, P, P]
+        L.oc_set_threads.argtypes = [ctypes.c_int]
+        L.oc_get_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
 

@@ -245,3 +245,34 @@ def test_tile_overflow_is_an_error():
     with pytest.raises(O.OracleError) as e:
         oc.update_x(1.0, 1.0, 0.0, 1.0)
     assert e.value.code == 1
+
+
+def test_parallel_oracle_equals_serial():
+    """The oracle's OpenMP loops (the reference's layered slabs, cells of the
+    kicks and PP) give the same bits on 1 and on several threads through a
+    full step: drift, PP, both deposits and kicks."""
+    from tests.helpers import zeldovich_state
+    nc, nnt = 24, 2
+    st = zeldovich_state(nc, nnt, seed=21, izipx=1, izipv=2)
+    out = []
+    nthr = O.lib().oc_get_threads()
+    try:
+        for threads in (1, 4):
+            O.lib().oc_set_threads(threads)
+            oc = O.OracleCube(nc, nnt, izipx=1, izipv=2, pp_on=True)
+            oc.set_state(st["xp"], st["vp"], st["pid"], st["rhoc"], st["vfield"])
+            oc.init_kernels()
+            oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+            vz = float(np.abs(st["vfield"][:, 2]).max() * 1.5 + 0.05)
+            s2, stats = oc.update_x(20.0, 20.0, vz, st["sigma_vi"])
+            oc.buffer_grid(); oc.buffer_x()
+            f2 = oc.ext_pp_force(st["a"], 20.0, st["sigma_vi"], st["mass_p"])
+            pm = oc.particle_mesh(st["a"], 20.0, st["sigma_vi"], s2, st["mass_p"])
+            out.append((oc.get_state(), s2, stats, f2, pm["vmax"].copy(), pm["testrhof"], pm["testrhoc"]))
+    finally:
+        O.lib().oc_set_threads(nthr)
+    a, b = out
+    for k in ("xp", "vp", "pid", "rhoc", "vfield"):
+        assert np.array_equal(a[0][k], b[0][k]), k
+    assert a[1] == b[1] and np.array_equal(a[2], b[2]) and a[3] == b[3]
+    assert np.array_equal(a[4], b[4]) and a[5] == b[5] and a[6] == b[6]
