@@ -15,8 +15,9 @@ particle_mesh -> buffer_v.  Inputs are resident on the GPU before timing.
 Prints ONE JSON line.  `roofline` is for the dominant kernel, from HIP-event
 timings on the library's stream; `step_roofline` is the whole step against
 the algorithmic bytes B_step = N_p*beta_p + N_f*54 + N_c*86; `cpu_baseline`
-times the oracle (the C restatement of the reference path, with scipy FFTs)
-on one host core on a 128^3-particle sample of the same step.
+times the oracle (the C restatement of the reference path, OpenMP, with
+scipy FFTs) on every host thread of this process's CPU share, one full step
+from the same 512^3 initial state.
 """
 import argparse
 import json
@@ -38,12 +39,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_state_gpu(g, box, z_i, seed):
-    """Zel'dovich ICs generated on the GPU straight into the context."""
+def make_state_gpu(g, box, z_i, seed, particles=True):
+    """Zel'dovich ICs generated on the GPU straight into the context (the
+    host copy of the state in `host`; particles=False: its cell arrays)."""
     t = time.time()
     a, sigma = g.make_ics(box, z_i, seed)
     log(f"[bench] GPU Zel'dovich ICs: {g.nplocal} particles in {time.time() - t:.1f}s, sigma_vi={sigma:.5g}")
-    st = g.get_state()
+    st = g.get_state(particles=particles)
     nc = g.nc
     # IC velocity constraint and vz_max (WU/initial_conditions.f90:631-641),
     # from the decoded cell velocities plus residuals bounded by the codec
@@ -51,7 +53,7 @@ def make_state_gpu(g, box, z_i, seed):
     vz = float(np.abs(vf[:, 2]).max()) + 3 * sigma * 2.5
     vmax = float(np.abs(vf).max()) + 3 * sigma * 2.5
     return dict(a=a, sigma_vi=sigma, np=st["nplocal"], vz_max=vz,
-                dt_vmax=float(np.float32(0.9) * np.float32(20) / np.float32(vmax)))
+                dt_vmax=float(np.float32(0.9) * np.float32(20) / np.float32(vmax)), host=st)
 
 
 class Step:
@@ -65,10 +67,13 @@ class Step:
         self.sigma = st["sigma_vi"]
         self.vz_max = st["vz_max"]
         self.dtc = dict(dt_fine=1000.0, dt_coarse=1000.0, dt_pp=1000.0, dt_vmax=st["dt_vmax"])
+        self.first = None  # (dt_old, dt, a_mid) of the first step, for the CPU baseline
 
     def __call__(self):
         g, c = self.g, self.clock
         c.timestep(**self.dtc)
+        if self.first is None:
+            self.first = dict(dt_old=float(c.dt_old), dt=float(c.dt), a_mid=float(c.a_mid))
         vs = g.update_x(float(c.dt_old), float(c.dt), self.vz_max, self.sigma)
         g.buffer_grid()
         g.buffer_x()
@@ -139,34 +144,109 @@ def pmc_traffic(stage, nc):
     return None
 
 
-def cpu_baseline(seconds_cap=60.0):
-    """Oracle step on a 128^3-particle sample (nc=64, 2^3 tiles, x1v2, PP on),
-    one host thread (scipy FFT workers=1)."""
+def host_cores():
+    """Threads this process may use on the host (its CPU share) and the CPU model."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_baseline(state, nc, nnt, box, dts, reps=1, izipx=1):
+    """The oracle (the C restatement of the reference path, OpenMP over the
+    reference's own parallel loops, scipy FFTs with the same thread count
+    standing in for threaded FFTW) on every host thread this process has,
+    on the same 512^3 initial state the GPU stepped: one full step
+    (update_x, buffers, PP, particle_mesh, buffer_v) per repetition, the
+    median reported, with per-stage times."""
     from oracle import oracle as O
-    from cafproject_amd import ics
-    O.FFT_WORKERS = 1
-    nc, nnt = 64, 2
-    x, v, a = ics.zeldovich(nc, 100.0, 49.0, seed=42)
-    st = ics.disjoint_state(x, v, nc, nnt, izipx=1, izipv=2)
-    n = x.shape[0]
-    oc = O.OracleCube(nc, nnt, izipx=1, izipv=2, pp_on=True, has_pid=False)
-    oc.set_state(st["xp"], st["vp"], None, st["rhoc"], st["vfield"])
+    threads, model = host_cores()
+    O.FFT_WORKERS = threads
+    O.lib().oc_set_threads(threads)
+    t = time.time()
+    oc = O.OracleCube(nc, nnt, izipx=izipx, izipv=2, pp_on=True, has_pid=False)
     oc.init_kernels()
-    oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+    n = int(state["nplocal"])
+    log(f"[bench] oracle context + Green's functions in {time.time() - t:.1f}s ({threads} threads, {model})")
     m = float(np.float32((4 * nc) ** 3 / n))
-    sig = st["sigma_vi"]
-    vz = float(np.abs(v[:, 2]).max())
+    sig = state["sigma_vi"]
+    times, stages = [], []
+    for r in range(reps):
+        oc.set_state(state["xp"], state["vp"], None, state["rhoc"], state["vfield"])
+        oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+        st = {}
+        t0 = time.perf_counter()
+        s2, _ = oc.update_x(dts["dt_old"], dts["dt"], state["vz_max"], sig)
+        t1 = time.perf_counter()
+        oc.buffer_grid(); oc.buffer_x()
+        t2 = time.perf_counter()
+        oc.ext_pp_force(dts["a_mid"], dts["dt"], sig, m)
+        t3 = time.perf_counter()
+        oc.particle_mesh(dts["a_mid"], dts["dt"], sig, s2, m)
+        t4 = time.perf_counter()
+        oc.buffer_v()
+        t5 = time.perf_counter()
+        st = dict(update_x=t1 - t0, buffers=(t2 - t1) + (t5 - t4), ext_pp_force=t3 - t2, particle_mesh=t4 - t3)
+        times.append(t5 - t0)
+        stages.append(st)
+        log(f"[bench] oracle step {r}: {t5 - t0:.1f} s " + " ".join(f"{k}={v:.2f}" for k, v in st.items()))
+    k = int(np.argsort(times)[len(times) // 2])
+    tmed = times[k]
+    return dict(value=n / tmed, unit="particle-updates/s", cores=threads, kind="port",
+                sample=f"one full oracle step (update_x, buffers, ext_pp_force, particle_mesh, buffer_v) of the "
+                       f"same {n}-particle initial state (nc={nc}, {nnt}^3 tiles, x{izipx}v2, PP on), median of "
+                       f"{reps}: {tmed:.1f} s on {threads} host threads ({model}); scipy FFTs on the same threads",
+                seconds=tmed, stage_seconds={a: round(b, 3) for a, b in stages[k].items()}, cpu_model=model)
+
+
+def large_box(steps=3, warmup=1):
+    """The large single-GPU box: 1024^3 particles, 800 Mpc/h, nc=512 (2048^3
+    fine mesh, compact layout), 4^3 tiles, int16 x and v, PP on, Zel'dovich
+    ICs at z=49 (seed 7): step time, particle-updates/s and each stage's
+    algorithmic GB/s (per step: in the compact layout the inverse FFT
+    stages launch once per force component)."""
+    from cafproject_amd.cube import Cube
+    nc, nnt, box = 512, 4, 800.0
+    t = time.time()
+    g = Cube(nc, nnt, izipx=2, izipv=2, has_pid=False, pp_on=True, pp_range=1, rsoft=0.3)
+    g.init_kernels()
+    st = make_state_gpu(g, box, 49.0, 7, particles=False)
+    del st["host"]
+    np_ = st["np"]
+    log(f"[bench] large box ready in {time.time() - t:.1f}s")
+    step = Step(g, st, float(np.float32((4 * nc) ** 3 / np_)))
+    for i in range(warmup):
+        step()
+    g.sync()
+    g.profile(True)
+    g.timers(reset=True)
     t0 = time.perf_counter()
-    s2, _ = oc.update_x(0.0, 0.5, vz, sig)
-    oc.buffer_grid(); oc.buffer_x()
-    oc.ext_pp_force(a, 0.5, sig, m)
-    oc.particle_mesh(a, 0.5, sig, s2, m)
-    oc.buffer_v()
-    t = time.perf_counter() - t0
-    return dict(value=n / t, unit="particle-updates/s", cores=1, kind="port",
-                sample=f"one oracle step (update_x, buffers, PP, particle_mesh) of {n} particles "
-                       f"(nc=64, 2^3 tiles, x1v2, PP on, z=49 ICs) in {t:.1f} s on 1 host thread",
-                seconds=t)
+    for i in range(steps):
+        step()
+    g.sync()
+    el = time.perf_counter() - t0
+    tm = g.timers()
+    g.profile(False)
+    ms = {k: v[0] / steps for k, v in tm.items()}
+    ms["fine_inverse"] = sum(ms[k] for k in ("fine_green", "fine_invy", "fine_c2r"))
+    sb = stage_bytes(np_, (4 * nc) ** 3, nc ** 3, 6, 6)
+    if tm["coarse_kick"][1] == 0 and tm["fine_kick"][1] > 0:
+        sb["fine_kick"] += sb.pop("coarse_kick")
+    stages = {k: dict(ms_per_step=round(ms[k], 3), bytes=int(b),
+                      gbps=round(b / (ms[k] * 1e-3) / 1e9, 1) if ms[k] > 0 else None) for k, b in sb.items()}
+    out = dict(workload="CUBE PM+PP step, 1024^3 particles, 800 Mpc/h, nc=512 (2048^3 fine mesh, compact layout), "
+                        "4^3 tiles, int16 x / int16 v, PP on, z=49 Zel'dovich EH no-wiggle, seed 7",
+               particles=np_, steps=steps, warmup=warmup, ms_per_step=1e3 * el / steps,
+               value=np_ * steps / el, unit="particle-updates/s", stages=stages, a_final=float(step.clock.a))
+    del step, g
+    return out
 
 
 def batched_gradients(iters=3):
@@ -215,9 +295,12 @@ def main():
     ap.add_argument("--nnt", type=int, default=2)
     ap.add_argument("--box", type=float, default=400.0)
     ap.add_argument("--no-pp", action="store_true")
+    ap.add_argument("--izipx", type=int, default=1, choices=(1, 2), help="bytes per position component")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-gradient ensemble measurement")
     ap.add_argument("--batched-iters", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=1, help="oracle steps timed for the CPU baseline (median)")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1024^3-particle large-box measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy kernel (torch), the reference "
                          "point for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
@@ -238,7 +321,7 @@ def main():
         del src, dst
         torch.cuda.empty_cache()
     nc, nnt = args.nc, args.nnt
-    izipx, izipv = 1, 2
+    izipx, izipv = args.izipx, 2
     pp_on = not args.no_pp
     t = time.time()
     g = Cube(nc, nnt, izipx=izipx, izipv=izipv, has_pid=False, pp_on=pp_on, pp_range=1, rsoft=0.3)
@@ -282,42 +365,54 @@ def main():
     per_launch = {k: dict(ms=round(stage_ms[k], 4), bytes=int(b),
                           gbps=round(b / (stage_ms[k] * 1e-3) / 1e9, 1) if stage_ms[k] > 0 else None)
                   for k, b in sb.items()}
-    # the dominant kernel among stages that run alone: an overlapped stage's
-    # event span includes the time it shares the GPU with other streams
-    single = [k for k in ("fine_deposit", "pp", "fine_kick", "coarse_deposit", "coarse_kick")
-              if k in per_launch and k not in CONCURRENT]
+    # the dominant kernel by time among the single-kernel stages; a stage
+    # that runs beside other streams' work (PP, the deposits) is timed by
+    # its event span, which includes the time it shares the GPU
+    single = [k for k in ("fine_deposit", "pp", "fine_kick", "coarse_deposit", "coarse_kick") if k in per_launch]
+
+    def kernel_entry(k):
+        d = per_launch[k]
+        ach = d["bytes"] / (d["ms"] * 1e-3)
+        tr = pmc_traffic(k, nc)
+        return {"bound": "hbm", "kernel": (ONE_PASS_KICKS if one_pass and k == "fine_kick" else MAIN_KERNEL[k]),
+                "stage": k, "concurrent_with": CONCURRENT.get(k),
+                "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
+                "traffic": tr, "traffic_source": os.path.relpath(PMC_TABLE, ROOT) if tr else None,
+                "algorithmic_bytes_per_launch": d["bytes"], "avg_launch_ms": d["ms"]}
+
     dom = max(single, key=lambda k: per_launch[k]["ms"])
-    d = per_launch[dom]
-    achieved = d["bytes"] / (d["ms"] * 1e-3)
+    roofline = kernel_entry(dom)
+    alone = [k for k in single if k not in CONCURRENT and k != dom]
+    if alone:
+        roofline["secondary"] = kernel_entry(max(alone, key=lambda k: per_launch[k]["ms"]))
     out = {
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"CUBE PM+PP step, {round(np_ ** (1 / 3))}^3 particles, {args.box:g} Mpc/h, "
-                               f"nc={nc} ({4 * nc}^3 fine mesh), {nnt}^3 tiles, int8 x / int16 v, "
+                               f"nc={nc} ({4 * nc}^3 fine mesh), {nnt}^3 tiles, int{8 * izipx} x / int16 v, "
                                f"PP {'on' if pp_on else 'off'}, z=49 Zel'dovich EH no-wiggle, seed 42",
                    "particles": np_, "nc": nc, "nnt": nnt, "nf": 4 * nc, "pp": pp_on,
                    "izipx": izipx, "izipv": izipv},
-        "roofline": {"bound": "hbm", "kernel": (ONE_PASS_KICKS if one_pass and dom == "fine_kick" else MAIN_KERNEL[dom]),
-                     "stage": dom,
-                     "overlapped_stages": {k: CONCURRENT[k] for k in per_launch if k in CONCURRENT},
-                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": pmc_traffic(dom, nc),
-                     "traffic_source": os.path.relpath(PMC_TABLE, ROOT) if pmc_traffic(dom, nc) else None,
-                     "algorithmic_bytes_per_launch": d["bytes"], "avg_launch_ms": d["ms"]},
+        "roofline": roofline,
         "step_roofline": {"B_step": B_step, "achieved_GBps": B_step / (ms_step * 1e-3) / 1e9,
                           "frac": B_step / (ms_step * 1e-3) / HBM_PEAK},
         "stages": per_launch,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "a_final": float(step.clock.a),
     }
+    first = step.first
+    del step, g
+    if not args.no_large:
+        out["large_box"] = large_box()
     if not args.no_batched:
-        del step, g
         out["batched"] = batched_gradients(args.batched_iters)
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline()
+            host = dict(st["host"], sigma_vi=st["sigma_vi"], vz_max=st["vz_max"])
+            out["cpu_baseline"] = cpu_baseline(host, nc, nnt, args.box, first, reps=args.cpu_reps, izipx=izipx)
+            out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
         except Exception as e:  # the baseline must not hide the GPU number
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)

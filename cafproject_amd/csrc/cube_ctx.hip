@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <new>
+#include <type_traits>
 
 extern "C" const char *cube_strerror(int code) {
   switch (code) {
@@ -56,11 +57,13 @@ int cube_slots_fold(cube_ctx *c, const unsigned long long *region, int nreg, int
 }
 
 static void free_all(cube_ctx *c) {
+  if (c->scratch_in_force) /* carved out of force_f, not allocations of their own */
+    c->dest = c->rank = c->xdest = c->xrank = nullptr, c->key = c->sortbuf = c->xkey = nullptr;
   void *ptrs[] = {c->xp, c->vp, c->xp2, c->vp2, c->pid, c->pid2, c->rhoc, c->rhoc2, c->vfield, c->vfield2,
                   c->cstart, c->cstart2, c->dest, c->key, c->sortbuf, c->fill, c->xdest, c->xkey, c->rank, c->xrank, c->fbflag,
                   c->tanf_tab, c->venc_tab, c->vdec0, c->vdec1, c->rho_f, c->spec_f, c->specd_f, c->force_f,
                   c->kern_f, c->rho_c, c->spec_c, c->specd_c, c->force_c, c->kern_c, c->dred,
-                  c->ured, c->ired, c->scanbuf, c->vpart, c->slots};
+                  c->ured, c->ired, c->scanbuf, c->vpart, c->slots, c->ppdefer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   cube_fft_destroy(c->plan_f);
@@ -143,6 +146,26 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   c->xcap = c->np_image / 64 + 4096;
   int st = CUBE_OK;
   const size_t xb = (size_t)p->izipx * 3, vb = (size_t)p->izipv * 3;
+  const int64_t nf3 = c->nf * c->nf * c->nf, nspec = (int64_t)cube_spec_pitch((int)c->nf) * c->nf * c->nf;
+  const int64_t ho = c->nf / 2 + 1;
+  /* Fine-mesh layout.  Full: three force spectra, the three components
+     solved together.  Compact (for meshes whose full layout does not fit,
+     e.g. 2048^3 in 288 GB): one force spectrum that the components pass
+     through one after the other, and the drift's sort scratch (dead outside
+     the drift and the initial count sort) inside the force mesh, behind
+     the room the initial conditions stage their velocities in.  Both give
+     the same bits. */
+  const size_t scratch_b = (size_t)c->cap * (4 + 8 + 8 + 4) + (size_t)c->xcap * (8 + 4 + 8 + 4) + 4096;
+  const size_t v_room = ((size_t)c->cap * 12 + 255) & ~(size_t)255;
+  int compact = p->mem_layout == 2;
+  if (p->mem_layout == 0) {
+    size_t fr = 0, tot = 0;
+    const size_t full = (size_t)c->cap * (2 * xb + 2 * vb + (p->has_pid ? 8 : 0)) + scratch_b +
+                        (size_t)c->ncells * 56 + (size_t)nf3 * 16 + (size_t)nspec * 32 +
+                        (size_t)cube_kernf_elems(ho) * 4 + (size_t)c->ncells * 64;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && full > fr / 100 * 97) compact = 1;
+  }
+  c->fine_nspec = compact ? 1 : 3;
   st |= dalloc_bytes(&c->xp, c->cap * xb);
   st |= dalloc_bytes(&c->vp, c->cap * vb);
   st |= dalloc_bytes(&c->xp2, c->cap * xb);
@@ -157,25 +180,40 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->vfield2, 3 * c->ncells);
   st |= dalloc(&c->cstart, c->ncells + 1);
   st |= dalloc(&c->cstart2, c->ncells + 1);
-  st |= dalloc(&c->dest, c->cap);
-  st |= dalloc(&c->key, c->cap);
-  st |= dalloc(&c->sortbuf, c->cap + c->xcap);
   st |= dalloc(&c->fill, c->ncells);
-  st |= dalloc(&c->xdest, c->xcap);
-  st |= dalloc(&c->xkey, c->xcap);
-  st |= dalloc(&c->rank, c->cap);
-  st |= dalloc(&c->xrank, c->xcap);
+  st |= dalloc(&c->ppdefer, c->ncells / 64 + 1);
   st |= dalloc(&c->fbflag, c->ncells / 64 + 2); /* one per 64-cell finalize workgroup of the drift */
   st |= dalloc(&c->tanf_tab, c->nvbin);
   st |= dalloc(&c->venc_tab, c->nvbin / 2);
   st |= dalloc(&c->vdec0, c->nvbin);
   st |= dalloc(&c->vdec1, c->nvbin);
-  const int64_t nf3 = c->nf * c->nf * c->nf, nspec = (int64_t)cube_spec_pitch((int)c->nf) * c->nf * c->nf;
-  const int64_t ho = c->nf / 2 + 1;
   st |= dalloc(&c->rho_f, nf3);
   st |= dalloc(&c->spec_f, nspec);
-  st |= dalloc(&c->specd_f, 3 * nspec);
+  st |= dalloc(&c->specd_f, c->fine_nspec * nspec);
   st |= dalloc(&c->force_f, 3 * nf3);
+  c->scratch_in_force = compact && !st && v_room + scratch_b <= (size_t)nf3 * 12;
+  if (c->scratch_in_force) {
+    char *b = reinterpret_cast<char *>(c->force_f) + v_room;
+    auto carve = [&b](auto **ptr, size_t n) {
+      *ptr = reinterpret_cast<std::remove_reference_t<decltype(**ptr)> *>(b);
+      b += (n * sizeof(**ptr) + 255) & ~(size_t)255;
+    };
+    carve(&c->key, c->cap);
+    carve(&c->sortbuf, c->cap + c->xcap);
+    carve(&c->xkey, c->xcap);
+    carve(&c->dest, c->cap);
+    carve(&c->rank, c->cap);
+    carve(&c->xdest, c->xcap);
+    carve(&c->xrank, c->xcap);
+  } else {
+    st |= dalloc(&c->dest, c->cap);
+    st |= dalloc(&c->key, c->cap);
+    st |= dalloc(&c->sortbuf, c->cap + c->xcap);
+    st |= dalloc(&c->xdest, c->xcap);
+    st |= dalloc(&c->xkey, c->xcap);
+    st |= dalloc(&c->rank, c->cap);
+    st |= dalloc(&c->xrank, c->xcap);
+  }
   st |= dalloc(&c->kern_f, cube_kernf_elems(ho));
   const int64_t nc3 = c->ncells, nspec_c = (int64_t)cube_spec_pitch((int)c->nc) * c->nc * c->nc;
   st |= dalloc(&c->rho_c, nc3);
@@ -404,7 +442,7 @@ static int pm_forces(cube_ctx *c, float mass_p, float a_mid, float dt, hipStream
     if (!st)
       st = cube_fft_green_c2r_3d(c, c->plan_f, c->spec_f, c->kern_f, 1, c->specd_f, c->force_f, (float)c->nf,
                                  nullptr, CUBE_T_FINE_GREEN, (unsigned *)c->ured,
-                                 c->slots + CUBE_SLOT_F2_FINE * CUBE_SLOTS, pre);
+                                 c->slots + CUBE_SLOT_F2_FINE * CUBE_SLOTS, pre, c->fine_nspec);
   }
   if (!st && hipEventRecord(c->ev_join, fs) != hipSuccess) st = CUBE_ERR_HIP;
   c->stream = cs;
@@ -491,7 +529,13 @@ extern "C" int cube_pm_fine_force(cube_ctx *c, float *f2) {
   tend(c, CUBE_T_FINE_FFT);
   return cube_fft_green_c2r_3d(c, c->plan_f, c->spec_f, c->kern_f, 1, c->specd_f, c->force_f, (float)c->nf, f2,
                                CUBE_T_FINE_GREEN, (unsigned *)c->ured,
-                               c->slots + CUBE_SLOT_F2_FINE * CUBE_SLOTS);
+                               c->slots + CUBE_SLOT_F2_FINE * CUBE_SLOTS, nullptr, c->fine_nspec);
+}
+extern "C" int cube_pm_fine_potential(cube_ctx *c, const float *wg1) {
+  if (c) c->pm_ahead = 0; /* the force mesh is overwritten */
+  if (!c || !wg1) return CUBE_ERR_INVALID;
+  CUBE_CHECK(hipSetDevice(c->device));
+  return cube_fine_potential(c, wg1);
 }
 extern "C" int cube_pm_fine_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new) {
   if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
@@ -598,6 +642,7 @@ static int field_ptr(cube_ctx *c, int which, void **ptr, size_t *bytes) {
     case CUBE_FIELD_FORCE_C: *ptr = c->force_c; *bytes = 3 * c->ncells * 4; return CUBE_OK;
     case CUBE_FIELD_KERN_F: *ptr = c->kern_f; *bytes = 3 * ho * ho * ho * 4; return CUBE_OK;
     case CUBE_FIELD_KERN_C: *ptr = c->kern_c; *bytes = 3 * (c->nc / 2 + 1) * c->nc * c->nc * 4; return CUBE_OK;
+    case CUBE_FIELD_PHI_F: *ptr = c->force_f; *bytes = nf3 * 4; return CUBE_OK;
   }
   return CUBE_ERR_INVALID;
 }
@@ -647,7 +692,7 @@ extern "C" int cube_set_field(cube_ctx *c, int which, const void *host, size_t b
   size_t n;
   int st = field_ptr(c, which, &p, &n);
   if (st) return st;
-  if (bytes != n) return CUBE_ERR_INVALID;
+  if (bytes != n || which == CUBE_FIELD_PHI_F) return CUBE_ERR_INVALID;
   CUBE_CHECK(hipSetDevice(c->device));
   CUBE_CHECK(hipStreamSynchronize(c->stream));
   if (which == CUBE_FIELD_KERN_F) {

@@ -157,6 +157,14 @@ template <int DIR> struct DFT<7, DIR> : DFTG<7, DIR> {};
  * buf[(j/NS)*NS*R + k + q*NS].  Values are staged in registers so the pass
  * is in place; the caller's barrier pattern is inside.
  */
+/* LDS index of element i of a line: one float2 of padding after every 32,
+   so the Stockham stores of stride R and R*NS float2 spread over the 64
+   banks of 4 bytes instead of piling on a few (the loads of consecutive
+   elements stay conflict free); a line of L elements then spans
+   L + L/32 + 1 float2 */
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lspan(int L) { return L + L / 32 + 1; }
+
 template <int L, int R, int NS, int DIR, int TL>
 __device__ __forceinline__ void stockham_pass(float2 *buf, int tl, const float2 *__restrict__ tw,
                                               int nt_tab) {
@@ -181,7 +189,7 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, int tl, const float2 
     const int j = tl + b * TL;
     if (PER * TL == NB || j < NB) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) v[b][q] = buf[j + q * NB];
+      for (int q = 0; q < R; ++q) v[b][q] = buf[lpad(j + q * NB)];
     }
   }
   __syncthreads();
@@ -201,7 +209,7 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, int tl, const float2 
       DFT<R, DIR>::run(v[b]);
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int q = 0; q < R; ++q) buf[base + q * NS] = v[b][q];
+      for (int q = 0; q < R; ++q) buf[lpad(base + q * NS)] = v[b][q];
     }
   }
   __syncthreads();
@@ -434,7 +442,7 @@ __global__ __launch_bounds__(RC<N>::NT) void k_c2c_r(float2 *__restrict__ a, con
 template <int N, bool OCT>
 __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict__ spec, float2 *__restrict__ outd,
                                                      const float *__restrict__ kern,
-                                                     const float2 *__restrict__ tw) {
+                                                     const float2 *__restrict__ tw, int d0, int d1) {
   using G = RC<N>;
   constexpr int NH = N / 2 + 1, H = N / 2, R1 = G::R1, R2 = G::R2, H1 = H + 1, SP = cube_spec_pitch(N);
   constexpr int64_t stride = (int64_t)N * SP, plane = (int64_t)SP * N * N;
@@ -470,7 +478,7 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
       const int ky = y <= H ? y : N - y;
       const float sy = y <= H ? 1.f : -1.f;
 #pragma unroll 1
-      for (int d = 0; d < 3; ++d) {
+      for (int d = d0; d < d1; ++d) {
         const int z0 = opaque_zero();
         float2 w[R2];
 #pragma unroll
@@ -494,7 +502,7 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
         }
         blk_fft<N, +1>(w, c, t, S, twl);
         if (ok) {
-          float2 *dst = outd + d * plane + (int64_t)y * SP;
+          float2 *dst = outd + (d - d0) * plane + (int64_t)y * SP;
           const int s1 = opaque_zero();
 #pragma unroll
           for (int m = 0; m < R2; ++m) *boff(uni(dst + (int64_t)(R1 * m) * stride, s1), lo) = w[m];
@@ -509,7 +517,7 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
 template <int N>
 __global__ __launch_bounds__(FFT_T) void k_r2c_x(const float *__restrict__ in, float2 *__restrict__ out,
                                                const float2 *__restrict__ tw, int64_t nrows) {
-  constexpr int L = N / 2, B = Cfg<N>::BC, TL = FFT_T / B, LP = L + 1;
+  constexpr int L = N / 2, B = Cfg<N>::BC, TL = FFT_T / B, LP = lspan(L);
   __shared__ float2 sm[B * LP];
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int64_t row0 = (int64_t)blockIdx.x * B;
@@ -526,15 +534,15 @@ __global__ __launch_bounds__(FFT_T) void k_r2c_x(const float *__restrict__ in, f
 #pragma unroll
   for (int i = 0; i < IT; ++i) {
     const int idx = tid + i * FFT_T, l = idx / L, m = idx % L;
-    if ((B * L) % FFT_T == 0 || idx < B * L) sm[l * LP + m] = ld[i];
+    if ((B * L) % FFT_T == 0 || idx < B * L) sm[l * LP + lpad(m)] = ld[i];
   }
   __syncthreads();
   line_fft<L, -1, TL>(sm + line * LP, tl, tw, N);
   /* split: X[k] = (Z[k]+conj(Z[L-k]))/2 + W^k (Z[k]-conj(Z[L-k]))/(2i) */
   for (int idx = tid; idx < B * (L + 1); idx += FFT_T) {
     const int l = idx / (L + 1), k = idx % (L + 1);
-    const float2 zk = sm[l * LP + (k == L ? 0 : k)];
-    const float2 zc = sm[l * LP + (k == 0 ? 0 : L - k)];
+    const float2 zk = sm[l * LP + lpad((k == L ? 0 : k))];
+    const float2 zc = sm[l * LP + lpad((k == 0 ? 0 : L - k))];
     const float2 e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y - zc.y));
     const float2 o = make_float2(0.5f * (zk.y + zc.y), -0.5f * (zk.x - zc.x));
     const float2 w = tw[k];
@@ -549,7 +557,7 @@ __global__ __launch_bounds__(FFT_T) void k_r2c_x(const float *__restrict__ in, f
 template <int N, int DIR>
 __global__ __launch_bounds__(FFT_T) void k_c2c_s(float2 *__restrict__ a, const float2 *__restrict__ tw,
                                                int ncol, int64_t stride, int64_t ostride) {
-  constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = N + 1;
+  constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = lspan(N);
   __shared__ float2 sm[B * NP];
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int col0 = blockIdx.x * B;
@@ -565,13 +573,13 @@ __global__ __launch_bounds__(FFT_T) void k_c2c_s(float2 *__restrict__ a, const f
 #pragma unroll
   for (int i = 0; i < IT; ++i) {
     const int idx = tid + i * FFT_T, l = idx % B, n = idx / B;
-    if ((B * N) % FFT_T == 0 || idx < B * N) sm[l * NP + n] = ld[i];
+    if ((B * N) % FFT_T == 0 || idx < B * N) sm[l * NP + lpad(n)] = ld[i];
   }
   __syncthreads();
   line_fft<N, DIR, TL>(sm + line * NP, tl, tw, N);
   for (int idx = tid; idx < B * N; idx += FFT_T) {
     const int l = idx % B, n = idx / B;
-    if (col0 + l < ncol) base[n * stride + l] = sm[l * NP + n];
+    if (col0 + l < ncol) base[n * stride + l] = sm[l * NP + lpad(n)];
   }
 }
 
@@ -583,8 +591,8 @@ __global__ __launch_bounds__(FFT_T) void k_c2c_s(float2 *__restrict__ a, const f
 template <int N>
 __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ spec, float2 *__restrict__ outd,
                                                  const float *__restrict__ kern, int octant,
-                                                 const float2 *__restrict__ tw) {
-  constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = N + 1, NH = N / 2 + 1, H = N / 2, SP = cube_spec_pitch(N);
+                                                 const float2 *__restrict__ tw, int d0, int d1) {
+  constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = lspan(N), NH = N / 2 + 1, H = N / 2, SP = cube_spec_pitch(N);
   constexpr int EPT = (B * N + FFT_T - 1) / FFT_T;
   __shared__ float2 sm[B * NP];
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
@@ -594,7 +602,7 @@ __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ sp
   const float2 *src = spec + (int64_t)y * SP + col0;
   for (int idx = tid; idx < B * N; idx += FFT_T) {
     const int l = idx % B, n = idx / B;
-    sm[l * NP + n] = (col0 + l < NH) ? src[n * stride + l] : make_float2(0.f, 0.f);
+    sm[l * NP + lpad(n)] = (col0 + l < NH) ? src[n * stride + l] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   line_fft<N, -1, TL>(sm + line * NP, tl, tw, N);
@@ -602,11 +610,11 @@ __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ sp
 #pragma unroll
   for (int s = 0; s < EPT; ++s) {
     const int idx = tid + s * FFT_T;
-    if (idx < B * N) keep[s] = sm[(idx % B) * NP + idx / B];
+    if (idx < B * N) keep[s] = sm[(idx % B) * NP + lpad(idx / B)];
   }
   const int ky = y <= H ? y : N - y;
   const float sy = y <= H ? 1.f : -1.f;
-  for (int d = 0; d < 3; ++d) {
+  for (int d = d0; d < d1; ++d) {
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < EPT; ++s) {
@@ -627,15 +635,15 @@ __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ sp
         }
         const float2 r = keep[s];
         /* rho_f(::2)=-crho(2::2)*K, rho_f(2::2)=crho(::2)*K  (W/pm.f90:119-120) */
-        sm[l * NP + n] = make_float2((-r.y) * K, r.x * K);
+        sm[l * NP + lpad(n)] = make_float2((-r.y) * K, r.x * K);
       }
     }
     __syncthreads();
     line_fft<N, +1, TL>(sm + line * NP, tl, tw, N);
-    float2 *dst = outd + d * plane + (int64_t)y * SP + col0;
+    float2 *dst = outd + (d - d0) * plane + (int64_t)y * SP + col0;
     for (int idx = tid; idx < B * N; idx += FFT_T) {
       const int l = idx % B, n = idx / B;
-      if (col0 + l < NH) dst[n * stride + l] = sm[l * NP + n];
+      if (col0 + l < NH) dst[n * stride + l] = sm[l * NP + lpad(n)];
     }
   }
 }
@@ -653,7 +661,7 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
      than 4 rows at 3 workgroups per CU despite idle threads in the passes,
      and faster than 4 rows one component at a time (every third float
      stored per pass: 11.4 against 8.7 ms at 1024^3) */
-  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = L + 1;
+  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = lspan(L);
   __shared__ float2 sm[3 * B * LP];
   __shared__ float red[FFT_T];
   /* a power-of-two divisor (every power-of-two mesh) has an exact inverse */
@@ -695,7 +703,7 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
         const float2 o = cmul(make_float2(a.x - c.x, a.y + c.y), w);
         z = make_float2(e.x - o.y, e.y + o.x);
       }
-      sm[(d * B + l) * LP + k] = z;
+      sm[(d * B + l) * LP + lpad(k)] = z;
     }
   }
   __syncthreads();
@@ -706,7 +714,7 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
     float f[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const float2 z = sm[(d * B + l) * LP + (x >> 1)];
+      const float2 z = sm[(d * B + l) * LP + lpad((x >> 1))];
       float v = (x & 1) ? z.y : z.x;
       if (pow2) { /* exact: v/2^k and v*2^-k round the same real number */
         v = v * inv;
@@ -751,11 +759,95 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
   if (tid == 0) atomicMax(cube_slot(f2slots), (unsigned long long)__float_as_uint(red[0]));
 }
 
+/* One force component of k_c2r_x3, for the compact fine mesh (one force
+   spectrum at a time): component d of the 3-float cells, /div three times.
+   Components 0 and 1 are stored as F; the pass of component 2 (last=1)
+   reads them back, forms max |F|^2 as k_c2r_x3 does and, with pre, stores
+   all three cells' kick factors (((F*a_mid)*dt)/6)/pi.  Every value goes
+   through the same line FFT and divisions as in k_c2r_x3, so the cells end
+   bit-identical to it. */
+template <int N>
+__global__ __launch_bounds__(FFT_T) void k_c2r_xc(const float2 *__restrict__ spec, float *__restrict__ force,
+                                                const float2 *__restrict__ tw, int64_t nrows, float div, int d,
+                                                int last, unsigned long long *__restrict__ f2slots, float pa,
+                                                float pdt, int pre) {
+  /* the rows per workgroup of k_c2r_x3, so the line FFTs are the same */
+  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = lspan(L);
+  __shared__ float2 sm[B * LP];
+  __shared__ float red[FFT_T];
+  const uint32_t db = __float_as_uint(div);
+  const bool pow2 = (db & 0x807FFFFFu) == 0u && (db >> 23) > 0u && (db >> 23) < 200u;
+  const float inv = 1.0f / div;
+  const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
+  const int64_t row0 = (int64_t)blockIdx.x * B;
+  for (int idx = tid; idx < B * L; idx += FFT_T) {
+    const int l = idx / L, k = idx % L;
+    float2 z = make_float2(0.f, 0.f);
+    if (row0 + l < nrows) {
+      const float2 a = spec[(row0 + l) * cube_spec_pitch(N) + k];
+      const float2 cc = spec[(row0 + l) * cube_spec_pitch(N) + (L - k)];
+      float2 w = tw[k];
+      w.y = -w.y;
+      const float2 e = make_float2(a.x + cc.x, a.y - cc.y);
+      const float2 o = cmul(make_float2(a.x - cc.x, a.y + cc.y), w);
+      z = make_float2(e.x - o.y, e.y + o.x);
+    }
+    sm[l * LP + lpad(k)] = z;
+  }
+  __syncthreads();
+  line_fft<L, +1, TL>(sm + line * LP, tl, tw, N);
+  float m = 0.f;
+  for (int idx = tid; idx < B * N; idx += FFT_T) {
+    const int l = idx / N, x = idx % N;
+    const float2 z = sm[l * LP + lpad((x >> 1))];
+    float v = (x & 1) ? z.y : z.x;
+    if (pow2) {
+      v = v * inv;
+      v = v * inv;
+      v = v * inv;
+    } else {
+      v = v / div;
+      v = v / div;
+      v = v / div;
+    }
+    if (row0 + l >= nrows) continue;
+    float *cell = force + ((row0 + l) * N + x) * 3;
+    if (!last) {
+      cell[d] = v;
+      continue;
+    }
+    const float f[3] = {cell[0], cell[1], v};
+    float o[3] = {f[0], f[1], f[2]};
+    if (pre) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float a = (f[q] * pa) * pdt;
+        bool ok = true;
+        const float gq = div_6pi(a, ok);
+        o[q] = ok ? gq : (a / 6.0f) / CUBE_PI_F;
+      }
+    }
+    cell[0] = o[0];
+    cell[1] = o[1];
+    cell[2] = o[2];
+    const float f2 = (f[0] * f[0] + f[1] * f[1]) + f[2] * f[2];
+    m = fmaxf(m, f2);
+  }
+  if (!last) return;
+  red[tid] = m;
+  __syncthreads();
+  for (int q = FFT_T / 2; q > 0; q >>= 1) {
+    if (tid < q) red[tid] = fmaxf(red[tid], red[tid + q]);
+    __syncthreads();
+  }
+  if (tid == 0) atomicMax(cube_slot(f2slots), (unsigned long long)__float_as_uint(red[0]));
+}
+
 /* single-component c2r (no Green's function), for inverse_full */
 template <int N>
 __global__ __launch_bounds__(FFT_T) void k_c2r_x1(const float2 *__restrict__ spec, float *__restrict__ out,
                                                 const float2 *__restrict__ tw, int64_t nrows) {
-  constexpr int L = N / 2, B = Cfg<N>::BC, TL = FFT_T / B, LP = L + 1;
+  constexpr int L = N / 2, B = Cfg<N>::BC, TL = FFT_T / B, LP = lspan(L);
   __shared__ float2 sm[B * LP];
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int64_t row0 = (int64_t)blockIdx.x * B;
@@ -771,20 +863,21 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x1(const float2 *__restrict__ spe
       const float2 o = cmul(make_float2(xk.x - xc.x, xk.y + xc.y), w);
       z = make_float2(e.x - o.y, e.y + o.x);
     }
-    sm[l * LP + k] = z;
+    sm[l * LP + lpad(k)] = z;
   }
   __syncthreads();
   line_fft<L, +1, TL>(sm + line * LP, tl, tw, N);
   for (int idx = tid; idx < B * N; idx += FFT_T) {
     const int l = idx / N, x = idx % N;
-    const float2 z = sm[l * LP + (x >> 1)];
+    const float2 z = sm[l * LP + lpad((x >> 1))];
     if (row0 + l < nrows) out[(row0 + l) * N + x] = (x & 1) ? z.y : z.x;
   }
 }
 
 /* ---------------- host side ---------------- */
 #define FFT_SIZES(X) \
-  X(12) X(16) X(18) X(20) X(24) X(32) X(48) X(64) X(72) X(80) X(96) X(128) X(192) X(256) X(512) X(1024) X(2048)
+  X(12) X(16) X(18) X(20) X(24) X(32) X(48) X(64) X(72) X(80) X(96) X(128) X(144) X(192) X(256) X(512) \
+      X(576) X(1024) X(2048)
 
 int cube_fft_supported(int64_t n) {
   switch (n) {
@@ -796,16 +889,20 @@ int cube_fft_supported(int64_t n) {
   return 0;
 }
 
+void cube_fft_twiddles(int n, float2 *h) {
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * 3.14159265358979323846 * (double)m / (double)n;
+    h[m] = make_float2((float)cos(a), (float)sin(a));
+  }
+}
+
 int cube_fft_create(cube_ctx *c, int n, FFTPlan **out) {
   (void)c;
   if (!cube_fft_supported(n)) return CUBE_ERR_INVALID;
   FFTPlan *p = new FFTPlan;
   p->n = n;
   float2 *h = new float2[n];
-  for (int m = 0; m < n; ++m) {
-    const double a = -2.0 * 3.14159265358979323846 * (double)m / (double)n;
-    h[m] = make_float2((float)cos(a), (float)sin(a));
-  }
+  cube_fft_twiddles(n, h);
   if (hipMalloc(&p->tw, n * sizeof(float2)) != hipSuccess) {
     delete[] h;
     delete p;
@@ -859,18 +956,22 @@ template <int N> struct Launch {
     constexpr int NH = N / 2 + 1;
     return strided<DIR>(c, p, a, (int64_t)cube_spec_pitch(N) * N, (int64_t)cube_spec_pitch(N), 1);
   }
-  static int green(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int oct, float2 *work3) {
+  /* components d0 <= d < d1, component d into work3 + (d-d0)*plane */
+  static int green(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int oct, float2 *work3,
+                   int d0 = 0, int d1 = 3) {
     constexpr int NH = N / 2 + 1;
     if constexpr (RC<N>::ok) {
       /* pairs of column blocks x the line pairs (y, N-y), y = 0..N/2 */
       dim3 g(((NH + RC<N>::CB - 1) / RC<N>::CB + 1) / 2, N / 2 + 1);
       if (oct)
-        hipLaunchKernelGGL((k_green_r<N, true>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw);
+        hipLaunchKernelGGL((k_green_r<N, true>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw, d0,
+                           d1);
       else
-        hipLaunchKernelGGL((k_green_r<N, false>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw);
+        hipLaunchKernelGGL((k_green_r<N, false>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw, d0,
+                           d1);
     } else {
       dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
-      hipLaunchKernelGGL(k_green_z<N>, g, dim3(FFT_T), 0, c->stream, spec, work3, kern, oct, p->tw);
+      hipLaunchKernelGGL(k_green_z<N>, g, dim3(FFT_T), 0, c->stream, spec, work3, kern, oct, p->tw, d0, d1);
     }
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
@@ -881,6 +982,15 @@ template <int N> struct Launch {
     const int64_t rows = (int64_t)N * N;
     hipLaunchKernelGGL(k_c2r_x3<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0, c->stream, work3,
                        force, p->tw, rows, div, f2, pa, pdt, pre);
+    CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
+  static int c2rc(cube_ctx *c, FFTPlan *p, const float2 *spec, float *force, float div, int d,
+                  unsigned long long *f2, float pa, float pdt, int pre) {
+    constexpr int BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC;
+    const int64_t rows = (int64_t)N * N;
+    hipLaunchKernelGGL(k_c2r_xc<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0,
+                       c->stream, spec, force, p->tw, rows, div, d, d == 2, f2, pa, pdt, pre);
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
   }
@@ -919,6 +1029,16 @@ int cube_fft_forward_full(cube_ctx *c, FFTPlan *p, const float *in, float2 *out)
   return CUBE_ERR_INVALID;
 }
 
+float2 *cube_fft_plan_twiddles(FFTPlan *p) { return p->tw; }
+
+int cube_fft_forward_full_tw(cube_ctx *c, int n, float2 *tw, const float *in, float2 *out) {
+  if (!cube_fft_supported(n)) return CUBE_ERR_INVALID;
+  FFTPlan p;
+  p.n = n;
+  p.tw = tw;
+  return cube_fft_forward_full(c, &p, in, out);
+}
+
 int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out) {
   switch (p->n) {
 #define CASE(N)                                                        \
@@ -937,10 +1057,12 @@ int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out) {
 /* spec: output of cube_fft_r2c_3d (x and y transformed).  Produces the
    three force components (3 floats per cell) and max |F|^2; with prescale
    = {a_mid, dt} the cells hold the fine kick's (((F*a_mid)*dt)/6)/pi
-   instead (max |F|^2 still of F). */
+   instead (max |F|^2 still of F).  work3 holds ncomp_spec spectra: 3 runs
+   the three components together, 1 one after the other through the same
+   spectrum (the compact fine mesh), with bit-identical results. */
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
                           float2 *work3, float *force3, float div, float *f2max, int t0, unsigned *f2dev,
-                          unsigned long long *f2slots, const float *prescale) {
+                          unsigned long long *f2slots, const float *prescale, int ncomp_spec) {
   const float pa = prescale ? prescale[0] : 0.f, pdt = prescale ? prescale[1] : 0.f;
   /* max |F|^2 as float bits: per workgroup into the slots, folded into the
      device word f2dev */
@@ -951,13 +1073,25 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
 #define CASE(N)                                                                   \
   case N: {                                                                       \
     int st;                                                                       \
-    if (t0 >= 0) tbeg(c);                                                         \
-    if ((st = Launch<N>::green(c, p, spec, kern, kern_octant, work3))) return st; \
-    if (t0 >= 0) { tend(c, t0); tbeg(c); }                                        \
-    if ((st = Launch<N>::c2c_y<+1>(c, p, work3, 3))) return st;                   \
-    if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                    \
-    if ((st = Launch<N>::c2r3(c, p, work3, force3, div, f2slots, pa, pdt, prescale != nullptr))) return st; \
-    if (t0 >= 0) tend(c, t0 + 2);                                                 \
+    if (ncomp_spec >= 3) {                                                        \
+      if (t0 >= 0) tbeg(c);                                                       \
+      if ((st = Launch<N>::green(c, p, spec, kern, kern_octant, work3))) return st; \
+      if (t0 >= 0) { tend(c, t0); tbeg(c); }                                      \
+      if ((st = Launch<N>::c2c_y<+1>(c, p, work3, 3))) return st;                 \
+      if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                  \
+      if ((st = Launch<N>::c2r3(c, p, work3, force3, div, f2slots, pa, pdt, prescale != nullptr))) return st; \
+      if (t0 >= 0) tend(c, t0 + 2);                                               \
+    } else {                                                                      \
+      for (int d = 0; d < 3; ++d) {                                               \
+        if (t0 >= 0) tbeg(c);                                                     \
+        if ((st = Launch<N>::green(c, p, spec, kern, kern_octant, work3, d, d + 1))) return st; \
+        if (t0 >= 0) { tend(c, t0); tbeg(c); }                                    \
+        if ((st = Launch<N>::c2c_y<+1>(c, p, work3, 1))) return st;               \
+        if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                \
+        if ((st = Launch<N>::c2rc(c, p, work3, force3, div, d, f2slots, pa, pdt, prescale != nullptr))) return st; \
+        if (t0 >= 0) tend(c, t0 + 2);                                             \
+      }                                                                           \
+    }                                                                             \
     break;                                                                        \
   }
     FFT_SIZES(CASE)

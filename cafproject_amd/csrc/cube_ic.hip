@@ -250,7 +250,7 @@ extern "C" int cube_set_state_xv(cube_ctx *c, int64_t n, const double *x, const 
   /* stage through the (idle) fine-mesh buffers */
   const size_t xb = (size_t)n * 3 * sizeof(double), vb = (size_t)n * 3 * sizeof(float);
   const size_t nspec = (size_t)cube_spec_pitch((int)c->nf) * c->nf * c->nf;
-  if (xb > 3 * nspec * sizeof(float2) || vb > 3 * (size_t)c->nf * c->nf * c->nf * sizeof(float))
+  if (xb > c->fine_nspec * nspec * sizeof(float2) || vb > 3 * (size_t)c->nf * c->nf * c->nf * sizeof(float))
     return CUBE_ERR_INVALID;
   double *dx = (double *)c->specd_f;
   float *dv = c->force_f;
@@ -266,6 +266,9 @@ extern "C" int cube_ic_zeldovich(cube_ctx *c, const cube_ic_params *ip, float *s
   const int ng = (int)(c->nc * c->p.np_nc);
   const int64_t n = (int64_t)ng * ng * ng;
   if (4 * c->nc % ng != 0 || n > c->cap) return CUBE_ERR_INVALID;
+  /* x (3 doubles) staged in the force spectra, v (3 floats) in the force mesh */
+  if ((size_t)n * 24 > (size_t)c->fine_nspec * cube_spec_pitch((int)c->nf) * c->nf * c->nf * sizeof(float2))
+    return CUBE_ERR_INVALID;
   FFTPlan *plan = nullptr;
   int st = cube_fft_create(c, ng, &plan);
   if (st) return st;

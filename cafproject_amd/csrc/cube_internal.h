@@ -73,7 +73,10 @@ struct cube_ctx {
   /* fine mesh */
   float *rho_f;             /* nf^3 */
   float2 *spec_f;           /* pitch*nf*nf */
-  float2 *specd_f;          /* 3 * pitch*nf*nf */
+  float2 *specd_f;          /* fine_nspec * pitch*nf*nf */
+  int fine_nspec;           /* force spectra held: 3, or 1 in the compact layout */
+  int scratch_in_force;     /* the drift's sort scratch lives inside force_f */
+  int32_t *ppdefer;         /* ncells/64: PP blocks deferred to the large-LDS launch */
   float *force_f;           /* 3*nf^3, (d fastest) */
   float *kern_f;            /* octant in 16-column blocks, cube_kernf_index */
   /* coarse mesh */
@@ -279,15 +282,21 @@ int cube_vdec_table(cube_ctx *c, double *dst, float sigma);
 int cube_scan_counts(cube_ctx *c, const int32_t *cnt, int64_t *start, int64_t n, int64_t *total);
 int cube_tile_boxsum(cube_ctx *c, const int32_t *rhoc, int w, int64_t *host_out);
 
-/* 1 when the line-FFT kernels are built for size n (12..2048, factors 2, 3) */
+/* 1 when the line-FFT kernels are built for size n (12..2048, factors 2, 3, 5) */
 int cube_fft_supported(int64_t n);
+/* the n twiddles exp(-2 pi i m/n) every plan of size n holds (host) */
+void cube_fft_twiddles(int n, float2 *h);
 int cube_fft_create(cube_ctx *c, int n, FFTPlan **out);
 void cube_fft_destroy(FFTPlan *p);
 int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
 int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
                           float2 *work3, float *force3, float scale_div, float *f2max, int timer0,
-                          unsigned *f2dev, unsigned long long *f2slots, const float *prescale = nullptr);
+                          unsigned *f2dev, unsigned long long *f2slots, const float *prescale = nullptr,
+                          int ncomp_spec = 3);
 int cube_fft_forward_full(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
+float2 *cube_fft_plan_twiddles(FFTPlan *p);
+/* the same with the size-n twiddles already on the device at tw (no plan) */
+int cube_fft_forward_full_tw(cube_ctx *c, int n, float2 *tw, const float *in, float2 *out);
 int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out);
 
 int cube_fine_deposit(cube_ctx *c, float mass_p, double *testrhof);
@@ -299,6 +308,8 @@ int cube_coarse_deposit(cube_ctx *c, float mass_p, double *testrhoc);
 int cube_coarse_kick(cube_ctx *c, float a_mid, float dt, float sigma_vi, float *vmax);
 int cube_pm_kicks(cube_ctx *c, float a_mid, float dt, float sigma_vi, float sigma_vi_new, float *vmax);
 int cube_green_init(cube_ctx *c, const float *wfxyzf, const float *wfxyzc);
+/* phi of the current fine density into force_f[0, nf^3) (cube_mesh.hip) */
+int cube_fine_potential(cube_ctx *c, const float *wg1);
 int cube_pp(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, float *f2_max_pp,
             int64_t *itest1);
 int cube_vstats_run(cube_ctx *c, float sigma_vi, double *sums3);

@@ -863,3 +863,70 @@ int cube_green_init(cube_ctx *c, const float *wfxyzf, const float *wfxyzc) {
   (void)hipFree(dctab);
   return st;
 }
+
+/* ---------------- fine potential ----------------
+ * The reference's main path never forms a potential (its fine force kernel
+ * goes from i*K*rho_k straight to forces); it ships the fine potential
+ * kernel table CUBEnu/kernels/wg1.ascii (CUBEnu/kernels/README:8-9), 16^3
+ * values at offsets 0..15.  Mirrored evenly into the periodic nf^3 mesh
+ * (zero beyond offset 15, the near corner zeroed with PP on as in the force
+ * kernel) its spectrum is real, and phi = C2R(Re K * R2C(rho_f)) / nf^3,
+ * divided by nf three times in real(4) like the force (W/pm.f90:122-123). */
+__global__ void k_fine_pot_rs(float *rs, int n, const float *tab, int pp_on, int pp_range) {
+  const int64_t n3 = (int64_t)n * n * n;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n3; q += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(q % n), y = (int)((q / n) % n), z = (int)(q / ((int64_t)n * n));
+    const int a0 = min(x, n - x), a1 = min(y, n - y), a2 = min(z, n - z);
+    float v = 0.f;
+    if (a0 <= 15 && a1 <= 15 && a2 <= 15 && !(pp_on && a0 <= pp_range && a1 <= pp_range && a2 <= pp_range))
+      v = tab[(a2 * 16 + a1) * 16 + a0];
+    rs[q] = v;
+  }
+}
+__global__ void k_mul_re(const float2 *spec, const float2 *kspec, float2 *out, int64_t n) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const float2 s = spec[q];
+    const float k = kspec[q].x;
+    out[q] = make_float2(s.x * k, s.y * k);
+  }
+}
+__global__ void k_div3(float *a, int64_t n3, float n) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n3; q += (int64_t)gridDim.x * blockDim.x) {
+    float v = a[q];
+    v = v / n;
+    v = v / n;
+    v = v / n;
+    a[q] = v;
+  }
+}
+
+int cube_fine_potential(cube_ctx *c, const float *wg1) {
+  const int nf = (int)c->nf;
+  const int64_t nf3 = (int64_t)nf * nf * nf, ns = (int64_t)cube_spec_pitch(nf) * nf * nf;
+  float *dtab = nullptr;
+  CUBE_CHECK(hipMalloc(&dtab, 16 * 16 * 16 * sizeof(float)));
+  int st = CUBE_OK;
+  if (hipMemcpy(dtab, wg1, 16 * 16 * 16 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = CUBE_ERR_HIP;
+  /* rho_k in spec_f; the kernel in force_f's first nf^3 floats, its
+     spectrum in force spectrum 0, the product back in spec_f; phi into
+     force_f's first nf^3 floats */
+  if (!st) st = cube_fft_forward_full(c, c->plan_f, c->rho_f, c->spec_f);
+  if (!st) {
+    hipLaunchKernelGGL(k_fine_pot_rs, dim3(2048), dim3(256), 0, c->stream, c->force_f, nf, dtab, c->p.pp_on,
+                       c->p.pp_range);
+    if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
+  }
+  if (!st) st = cube_fft_forward_full(c, c->plan_f, c->force_f, c->specd_f);
+  if (!st) {
+    hipLaunchKernelGGL(k_mul_re, dim3(2048), dim3(256), 0, c->stream, c->spec_f, c->specd_f, c->spec_f, ns);
+    if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
+  }
+  if (!st) st = cube_fft_inverse_full(c, c->plan_f, c->spec_f, c->force_f);
+  if (!st) {
+    hipLaunchKernelGGL(k_div3, dim3(2048), dim3(256), 0, c->stream, c->force_f, nf3, (float)nf);
+    if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dtab);
+  return st;
+}

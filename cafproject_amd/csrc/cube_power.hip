@@ -116,22 +116,26 @@ __global__ void k_pw_zero(double *a, int64_t n) {
     a[q] = 0.0;
 }
 
-extern "C" int cube_power(cube_ctx *c, int32_t ng, int32_t nbin, double *ksum, double *psum, int64_t *count) {
-  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
+/* Checks shared by both entry points. */
+static int power_args(cube_ctx *c, int32_t ng, int32_t nbin, const double *ksum, const double *psum,
+                      const int64_t *count) {
   if (!c || !ksum || !psum || !count || ng < 4 || ng > c->nf || c->nf % ng || nbin < 1) return CUBE_ERR_INVALID;
   if (!c->has_state) return CUBE_ERR_STATE;
   if (c->nt > 519) return CUBE_ERR_INVALID; /* k_pw_deposit stages nt+1 row offsets in LDS */
-  CUBE_CHECK(hipSetDevice(c->device));
-  const int64_t n3 = (int64_t)ng * ng * ng;
-  /* scratch: the FP64 mesh in the force spectra, the FP32 delta in rho_f,
-     its spectrum in spec_f, the bin sums in the force mesh */
-  double *rho = reinterpret_cast<double *>(c->specd_f);
-  double *bins = reinterpret_cast<double *>(c->force_f); /* 3*(nbin+1) */
-  if ((size_t)(3 * (nbin + 1)) * sizeof(double) > (size_t)3 * c->nf * c->nf * c->nf * sizeof(float))
+  /* the FP64 mesh lives in the force spectra */
+  if ((size_t)ng * ng * ng * sizeof(double) >
+      (size_t)c->fine_nspec * cube_spec_pitch((int)c->nf) * c->nf * c->nf * sizeof(float2))
     return CUBE_ERR_INVALID;
-  FFTPlan *plan = nullptr;
-  int st = cube_fft_create(c, ng, &plan);
-  if (st) return st;
+  return CUBE_OK;
+}
+
+/* Deposit, contrast, forward FFT and binning on the context's stream: the
+   FP64 mesh in the force spectra, the FP32 delta in rho_f, its spectrum in
+   spec_f; bins = 3*(nbin+1) doubles (sum |k|, sum Delta^2, count), cleared
+   here by a fill kernel. */
+static int power_device(cube_ctx *c, int32_t ng, int32_t nbin, float2 *tw, double *bins) {
+  const int64_t n3 = (int64_t)ng * ng * ng;
+  double *rho = reinterpret_cast<double *>(c->specd_f);
   hipLaunchKernelGGL(k_pw_zero, dim3(1024), dim3(256), 0, c->stream, rho, n3);
   hipLaunchKernelGGL(k_pw_zero, dim3(1), dim3(256), 0, c->stream, bins, (int64_t)3 * (nbin + 1));
   const int64_t nrows = c->ncells / c->nt;
@@ -143,14 +147,28 @@ extern "C" int cube_power(cube_ctx *c, int32_t ng, int32_t nbin, double *ksum, d
                        c->cstart, geo_of(c), ng, rho);
   hipLaunchKernelGGL(k_pw_delta, dim3(1024), dim3(256), 0, c->stream, rho, n3, (double)n3 / (double)c->nplocal,
                      c->rho_f);
-  if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
-  if (!st) st = cube_fft_forward_full(c, plan, c->rho_f, c->spec_f);
-  if (!st) {
-    const size_t lds = (size_t)(nbin + 1) * (2 * sizeof(double) + sizeof(unsigned long long));
-    hipLaunchKernelGGL(k_pw_bin, dim3(512), dim3(PW_T), lds, c->stream, c->spec_f, ng, nbin, bins,
-                       bins + (nbin + 1), (unsigned long long *)(bins + 2 * (nbin + 1)));
-    if (hipGetLastError() != hipSuccess) st = CUBE_ERR_HIP;
-  }
+  if (hipGetLastError() != hipSuccess) return CUBE_ERR_HIP;
+  int st = cube_fft_forward_full_tw(c, ng, tw, c->rho_f, c->spec_f);
+  if (st) return st;
+  const size_t lds = (size_t)(nbin + 1) * (2 * sizeof(double) + sizeof(unsigned long long));
+  hipLaunchKernelGGL(k_pw_bin, dim3(512), dim3(PW_T), lds, c->stream, c->spec_f, ng, nbin, bins, bins + (nbin + 1),
+                     (unsigned long long *)(bins + 2 * (nbin + 1)));
+  return hipGetLastError() != hipSuccess ? CUBE_ERR_HIP : CUBE_OK;
+}
+
+extern "C" int cube_power(cube_ctx *c, int32_t ng, int32_t nbin, double *ksum, double *psum, int64_t *count) {
+  if (c) c->pm_ahead = 0; /* forces formed ahead by ext_pp_force no longer apply */
+  int st = power_args(c, ng, nbin, ksum, psum, count);
+  if (st) return st;
+  CUBE_CHECK(hipSetDevice(c->device));
+  /* the bin sums in the force mesh, the twiddles in a plan of their own */
+  double *bins = reinterpret_cast<double *>(c->force_f);
+  if ((size_t)(3 * (nbin + 1)) * sizeof(double) > (size_t)3 * c->nf * c->nf * c->nf * sizeof(float))
+    return CUBE_ERR_INVALID;
+  FFTPlan *plan = nullptr;
+  st = cube_fft_create(c, ng, &plan);
+  if (st) return st;
+  st = power_device(c, ng, nbin, cube_fft_plan_twiddles(plan), bins);
   if (!st) {
     double *h = new double[3 * (nbin + 1)];
     if (hipMemcpyAsync(h, bins, 3 * (nbin + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||

@@ -142,9 +142,15 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
  *
  * A fine cell lies in exactly one coarse cell, so one thread per staged
  * coarse cell counts its particles into the per-fine-cell table and, after
- * the scan, places them walking l downwards, with plain LDS updates.  A
- * block whose neighbourhood holds more than `maxs` particles walks the
- * global lists instead.
+ * the scan, places them walking l downwards, with plain LDS updates.
+ *
+ * Two launches share the kernel.  The first, one workgroup per block with
+ * a small staging area (several workgroups per CU), defers a block whose
+ * neighbourhood holds more than `maxs` particles -- a clustered region --
+ * to the list `dlist`.  The second walks that list with a staging area of
+ * most of a CU's LDS (one workgroup per CU, `list`/`nlist`); a block that
+ * overflows even that walks the global lists instead.  Every block's
+ * particles are kicked exactly once, by the same arithmetic either way.
  */
 #define PPB_T 256
 __device__ static inline int pp_fix(int IX, int cc, int32_t code) {
@@ -161,7 +167,10 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
                                                   const int64_t *__restrict__ cstart,
                                                   const double *__restrict__ vdec, const double *__restrict__ vtab, Geo g, int nvh, int pr,
                                                   double rsoft, double mass, double a_mid, double dt, double venc,
-                                                  int maxs, unsigned long long *__restrict__ out) {
+                                                  int maxs, unsigned long long *__restrict__ out,
+                                                  int32_t *__restrict__ dlist, unsigned long long *__restrict__ dcount,
+                                                  const int32_t *__restrict__ list,
+                                                  const unsigned long long *__restrict__ nlist) {
   extern __shared__ uint64_t smem64[];
   const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
   uint64_t *pos = smem64;                                            /* packed positions (maxs) */
@@ -171,7 +180,12 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   __shared__ int rowoff[17];
   const int tid = threadIdx.x, nt = g.nt, nb = nt / 4, nnt = g.nnt;
   const int sh = IX == 1 ? 6 : 14;
-  int64_t b = blockIdx.x;
+  double f2m = 0;
+  unsigned long long ntest = 0;
+  /* first launch: block blockIdx.x; second launch: the deferred blocks */
+  const int64_t nw = list ? (int64_t)*nlist : (int64_t)gridDim.x;
+  for (int64_t w = blockIdx.x; w < nw; w += gridDim.x) {
+  const int64_t b = list ? (int64_t)list[w] : w;
   const int bx = (int)(b % nb), by = (int)((b / nb) % nb), bz = (int)((b / ((int64_t)nb * nb)) % nb);
   const int t = (int)(b / ((int64_t)nb * nb * nb));
   const int tt[3] = {t % nnt, (t / nnt) % nnt, t / (nnt * nnt)};
@@ -242,6 +256,11 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   const int incl = block_incl_scan<PPB_T / 64>(s, part, nstage);
   const bool staged = nstage <= maxs; /* uniform over the block */
   s = incl - s;
+  if (!staged && dlist) { /* uniform: the block goes to the second launch */
+    if (tid == 0) dlist[atomicAdd(dcount, 1ull)] = (int32_t)b;
+    __syncthreads();
+    continue;
+  }
   for (int i = i0; i < i1; ++i) {
     const int cnt = cur[i];
     cur[i] = (uint16_t)(staged ? s : 0);
@@ -279,8 +298,6 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   __syncthreads();
   /* fine cell f's list is pos[cur[f-1] .. cur[f]) */
 
-  double f2m = 0;
-  unsigned long long ntest = 0;
   const int ntot = rowoff[16];
   for (int i = tid; i < ntot; i += PPB_T) {
     int row = 0; /* the last of the 16 rows with rowoff[row] <= i */
@@ -401,6 +418,8 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     f2m = f2 > f2m ? f2 : f2m;
     ++ntest;
   }
+  __syncthreads(); /* the block's tables are read until here */
+  }
   /* max |f|^2 and the particle count: wave shuffles, the waves through
      LDS, one global atomic pair per workgroup (max and sum do not depend
      on the order) */
@@ -426,26 +445,54 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   }
 }
 
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 template <int IX, int IV>
 static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, unsigned long long *out) {
   const int pr = (int)c->p.pp_range;
   if (c->nt % 4 == 0 && pr >= 0 && pr <= 4) {
-    /* 64 KB of LDS per workgroup: the fine-cell table, the staged positions
-       (3 ints each) and the walked particles' cell indices (2 per int) */
     const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
-    /* staged positions (8 bytes each) and the uint16 fine-cell table in
-       about 28 KB, so five workgroups share a CU's LDS */
+    const size_t table = (((size_t)nreg + 1) * 2 + 7) / 8 * 8; /* uint16 per fine cell, plus cur[-1] */
+    /* first launch: staged positions (8 bytes each) and the fine-cell
+       table in about 28 KB, so five workgroups share a CU's LDS */
     int maxs = pr <= 1 ? 2048 : (pr == 2 ? 1024 : 512);
-    /* a lower cap sends more workgroups down the unstaged walk (used by the
-       tests to cover both walks in one run) */
+    /* second launch: most of the LDS a workgroup may take, for the blocks
+       the first one deferred (uint16 offsets cap it at 65535) */
+    int lds = 65536;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess) lds = 65536;
+    int maxb = (int)std::min<int64_t>(16384, ((int64_t)lds - 4096 - (int64_t)table) / 8);
+    /* lower caps send more workgroups down the second launch or the
+       unstaged walk (used by the tests to cover every path in one run) */
     if (const char *e = getenv("CUBE_PP_STAGE_MAX")) maxs = std::min(maxs, atoi(e));
+    if (const char *e = getenv("CUBE_PP_BIG_MAX")) maxb = std::min(maxb, atoi(e));
+    const bool two = maxb > maxs;
     const int64_t nb = c->nt / 4, nblk = nb * nb * nb * c->ntiles;
-    const size_t dyn = (size_t)maxs * 8 + (((size_t)nreg + 1) * 2 + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3((unsigned)nblk), dim3(PPB_T), dyn, c->stream,
+    unsigned long long *dcount = (unsigned long long *)c->ured + 24;
+    if (two) {
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, dcount, 1);
+      CUBE_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3((unsigned)nblk), dim3(PPB_T), (size_t)maxs * 8 + table, c->stream,
                        (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
                        c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p, (double)a_mid,
-                       (double)dt, cube_vconst(sigma_vi), maxs, out);
+                       (double)dt, cube_vconst(sigma_vi), maxs, out, two ? c->ppdefer : nullptr, dcount, nullptr,
+                       nullptr);
     CUBE_LAUNCH_CHECK();
+    if (two) {
+      /* a fixed grid of one workgroup per CU-slot walks however many blocks
+         were deferred (read on the device: no host round trip) */
+      int ncu = 256;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) ncu = 256;
+      const unsigned grid = (unsigned)std::min<int64_t>(nblk, 2 * (int64_t)ncu);
+      hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3(grid), dim3(PPB_T), (size_t)maxb * 8 + table, c->stream,
+                         (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
+                         c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p,
+                         (double)a_mid, (double)dt, cube_vconst(sigma_vi), maxb, out, nullptr, nullptr,
+                         c->ppdefer, dcount);
+      CUBE_LAUNCH_CHECK();
+    }
     return CUBE_OK;
   }
   const int64_t nrows = c->ncells / c->nt;

@@ -25,7 +25,7 @@ ERR_NAMES = {
     1: "TILE_OVERFLOW", 2: "IMAGE_OVERFLOW", 3: "PP_OVERFLOW", 4: "CHECKSUM", 5: "COUNT",
     6: "DISPLACEMENT", 10: "HIP", 11: "ALLOC", 12: "INVALID", 13: "STATE",
 }
-FIELD = dict(rho_f=0, force_f=1, rho_c=2, force_c=3, kern_f=4, kern_c=5)
+FIELD = dict(rho_f=0, force_f=1, rho_c=2, force_c=3, kern_f=4, kern_c=5, phi_f=6)
 
 
 class CubeError(RuntimeError):
@@ -40,7 +40,7 @@ class Params(ctypes.Structure):
                 ("has_pid", ctypes.c_int32), ("pp_on", ctypes.c_int32), ("pp_range", ctypes.c_int32),
                 ("rsoft", ctypes.c_float), ("image_buffer", ctypes.c_float),
                 ("tile_buffer", ctypes.c_float), ("device", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("mem_layout", ctypes.c_int32)]
 
 
 class VStats(ctypes.Structure):
@@ -79,7 +79,8 @@ EXPORTS = ["cube_ctx_create", "cube_ctx_destroy", "cube_strerror", "cube_ctx_inf
            "cube_pp_force", "cube_particle_mesh", "cube_buffer_v", "cube_pm_fine_deposit",
            "cube_pm_fine_force", "cube_pm_fine_kick", "cube_pm_coarse_deposit", "cube_pm_coarse_force",
            "cube_pm_coarse_kick", "cube_get_field", "cube_set_field", "cube_sync", "cube_host_atan",
-           "cube_profile", "cube_timers", "cube_ic_zeldovich", "cube_set_state_xv", "cube_power"]
+           "cube_profile", "cube_timers", "cube_ic_zeldovich", "cube_set_state_xv", "cube_power",
+           "cube_pm_fine_potential"]
 TIMERS = ["drift", "vstats", "pp", "fine_deposit", "fine_fft", "fine_green", "fine_invy", "fine_c2r",
           "fine_kick", "coarse_deposit", "coarse_force", "coarse_kick", "checks"]
 
@@ -117,6 +118,7 @@ def lib():
         L.cube_particle_mesh.argtypes = [P, f32, f32, f32, f32, f32, ctypes.POINTER(PMOut)]
         L.cube_pm_fine_deposit.argtypes = [P, f32, P]
         L.cube_pm_fine_force.argtypes = [P, P]
+        L.cube_pm_fine_potential.argtypes = [P, P]
         L.cube_pm_fine_kick.argtypes = [P, f32, f32, f32, f32]
         L.cube_pm_coarse_deposit.argtypes = [P, f32, P]
         L.cube_pm_coarse_force.argtypes = [P, P]
@@ -135,6 +137,19 @@ def lib():
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def load_potential_table(datadir=None):
+    """The fine potential kernel table CUBEnu/kernels/wg1.ascii (16^3 lines
+    `i j k value`, i fastest) as 4096 float32 values in (i,j,k) order."""
+    d = datadir or os.path.join(HERE, "data")
+    t = np.loadtxt(os.path.join(d, "wg1.ascii"))
+    idx = t[:, :3].astype(int)
+    want = np.stack(np.meshgrid(np.arange(1, 17), np.arange(1, 17), np.arange(1, 17), indexing="ij"),
+                    -1)[..., ::-1].reshape(-1, 3)
+    if not np.array_equal(idx, want):
+        raise ValueError("wg1.ascii: index mismatch")
+    return np.ascontiguousarray(t[:, 3], np.float32)
 
 
 def load_tables(datadir=None):
@@ -163,10 +178,10 @@ class Cube:
     """One image (the whole periodic box) of the CUBE state on one GPU."""
 
     def __init__(self, nc, nnt, ncb=6, np_nc=2, izipx=2, izipv=2, has_pid=True, pp_on=False,
-                 pp_range=1, rsoft=0.3, image_buffer=1.5, tile_buffer=2.5, device=0):
+                 pp_range=1, rsoft=0.3, image_buffer=1.5, tile_buffer=2.5, device=0, mem_layout=0):
         L = lib()
         self.p = Params(nc, nnt, ncb, np_nc, izipx, izipv, int(has_pid), int(pp_on), pp_range, rsoft,
-                        image_buffer, tile_buffer, device, 0)
+                        image_buffer, tile_buffer, device, mem_layout)
         h = ctypes.c_void_p()
         self._check(L.cube_ctx_create(ctypes.byref(self.p), ctypes.byref(h)), "cube_ctx_create")
         self._h = h
@@ -246,13 +261,15 @@ class Cube:
         self._check(lib().cube_ic_zeldovich(self._h, ctypes.byref(ip), ctypes.byref(s)), "make_ics")
         return a, s.value
 
-    def get_state(self):
+    def get_state(self, particles=True):
+        """The disjoint state (cube_get_state); particles=False fetches only
+        the cell arrays rhoc and vfield."""
         n = self.nplocal
         xt = np.int8 if self.izipx == 1 else np.int16
         vt = np.int8 if self.izipv == 1 else np.int16
-        xp = np.zeros((n, 3), xt)
-        vp = np.zeros((n, 3), vt)
-        pid = np.zeros(n, np.int32) if self.has_pid else None
+        xp = np.zeros((n, 3), xt) if particles else None
+        vp = np.zeros((n, 3), vt) if particles else None
+        pid = np.zeros(n, np.int32) if self.has_pid and particles else None
         rhoc = np.zeros(self.nc ** 3, np.int32)
         vf = np.zeros((self.nc ** 3, 3), np.float32)
         nn = ctypes.c_int64()
@@ -293,6 +310,14 @@ class Cube:
         self._check(lib().cube_pm_fine_deposit(self._h, mass_p, ctypes.byref(t)), "fine_deposit")
         return t.value
 
+    def fine_potential(self, wg1=None):
+        """Fine potential of the current fine density (cube_pm_fine_potential),
+        returned as [z][y][x]."""
+        if wg1 is None:
+            wg1 = load_potential_table()
+        self._check(lib().cube_pm_fine_potential(self._h, _ptr(wg1)), "fine_potential")
+        return self.get_field("phi_f")
+
     def fine_force(self):
         f = ctypes.c_float()
         self._check(lib().cube_pm_fine_force(self._h, ctypes.byref(f)), "fine_force")
@@ -322,7 +347,7 @@ class Cube:
         ho = nf // 2 + 1
         return dict(rho_f=(nf, nf, nf), force_f=(nf, nf, nf, 3), rho_c=(nc, nc, nc),
                     force_c=(nc, nc, nc, 3), kern_f=(3, ho, ho, ho),
-                    kern_c=(3, nc, nc, nc // 2 + 1))[name]
+                    kern_c=(3, nc, nc, nc // 2 + 1), phi_f=(nf, nf, nf))[name]
 
     def get_field(self, name):
         a = np.zeros(self.field_shape(name), np.float32)

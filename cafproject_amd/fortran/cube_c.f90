@@ -13,7 +13,7 @@ module cube_c
     integer(c_int64_t) :: nc, nnt, ncb, np_nc
     integer(c_int32_t) :: izipx, izipv, has_pid, pp_on, pp_range
     real(c_float) :: rsoft, image_buffer, tile_buffer
-    integer(c_int32_t) :: device, reserved
+    integer(c_int32_t) :: device, mem_layout
   end type
 
   ! outputs of update_x (W/update_particle.f90:172-250)

@@ -94,7 +94,7 @@ program cube_main
   p%izipx = int(hi(5), 4); p%izipv = int(hi(6), 4)
   p%has_pid = merge(1, 0, has_pid); p%pp_on = pp_arg; p%pp_range = 1
   p%rsoft = 0.3; p%image_buffer = 1.5; p%tile_buffer = 2.5
-  p%device = 0; p%reserved = 0
+  p%device = 0; p%mem_layout = 0
   call cube_check(cube_ctx_create(p, ctx), 'cube_ctx_create')
   call cube_check(cube_init_kernels(ctx, wf, wc), 'cube_init_kernels')
   call cube_check(cube_set_state(ctx, nplocal, npglobal, pxp, pvp, ppid, c_loc(rhoc), c_loc(vfield)), &

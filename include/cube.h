@@ -74,7 +74,10 @@ typedef struct {
   float image_buffer;/* reference image_buffer (1.5)                      */
   float tile_buffer; /* reference tile_buffer (2.5)                       */
   int32_t device;    /* HIP device ordinal                                */
-  int32_t reserved;
+  int32_t mem_layout;/* fine mesh layout: 0 = compact when the full one
+                        does not fit the device's free memory, 1 = full,
+                        2 = compact (one force spectrum at a time, drift
+                        scratch inside the force mesh; same results)     */
 } cube_params;
 
 /* Outputs of update_x (W/update_particle.f90:172-250). */
@@ -138,6 +141,12 @@ int cube_pm_fine_deposit(cube_ctx *ctx, float mass_p, double *testrhof);
 int cube_pm_fine_force(cube_ctx *ctx, float *f2_max_fine);
 int cube_pm_fine_kick(cube_ctx *ctx, float a_mid, float dt, float sigma_vi,
                       float sigma_vi_new);
+/* Fine potential of the current fine density (after cube_pm_fine_deposit),
+   read with CUBE_FIELD_PHI_F.  The reference's step never forms one; it
+   ships the fine potential kernel CUBEnu/kernels/wg1.ascii (README:8-9):
+   wg1 = its 16^3 values, (i,j,k) order with i fastest, mirrored evenly
+   into the periodic nf^3 mesh; phi = C2R(K * R2C(rho_f)) / nf^3. */
+int cube_pm_fine_potential(cube_ctx *ctx, const float *wg1);
 int cube_pm_coarse_deposit(cube_ctx *ctx, float mass_p, double *testrhoc);
 int cube_pm_coarse_force(cube_ctx *ctx, float *f2_max_coarse);
 int cube_pm_coarse_kick(cube_ctx *ctx, float a_mid, float dt, float sigma_vi, float *vmax);
@@ -176,7 +185,10 @@ enum {
   CUBE_FIELD_RHO_C = 2,   /* nc^3 float, coarse density                             */
   CUBE_FIELD_FORCE_C = 3, /* nc^3 x 3 float, coarse force                           */
   CUBE_FIELD_KERN_F = 4,  /* 3 x (nf/2+1)^3 float, fine Green's function octant     */
-  CUBE_FIELD_KERN_C = 5   /* 3 x nc x nc x (nc/2+1) float, coarse Green's function  */
+  CUBE_FIELD_KERN_C = 5,  /* 3 x nc x nc x (nc/2+1) float, coarse Green's function  */
+  CUBE_FIELD_PHI_F = 6    /* nf^3 float, fine potential of cube_pm_fine_potential
+                              (read only; it shares the force mesh's memory, so
+                              any later call that forms forces replaces it)   */
 };
 int cube_get_field(cube_ctx *ctx, int which, void *host, size_t bytes);
 int cube_set_field(cube_ctx *ctx, int which, const void *host, size_t bytes);

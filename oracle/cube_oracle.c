@@ -1106,3 +1106,51 @@ void oc_expansion_n(int64_t n, const float *a0, const float *dt0, float om, floa
 /* host threads of the oracle's parallel loops (OpenMP) */
 void oc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int oc_get_threads(void) { return omp_get_max_threads(); }
+
+/* ---------------- FP32 mesh glue of the force solves ----------------
+ * The element-wise steps around the FFTs of W/pm.f90:119-126 and :285-315,
+ * threaded like the reference's OpenMP build: each is one float operation
+ * per element, so the result does not depend on the thread count. */
+
+/* rho_f(::2) = -crho(2::2)*kern, rho_f(2::2) = crho(::2)*kern
+   (W/pm.f90:119-120, :281-282): complex64 interleaved in and out */
+void oc_green_mul(const float *crho, const float *kern, float *out, int64_t n) {
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < n; ++q) {
+    const float re = crho[2 * q], im = crho[2 * q + 1], k = kern[q];
+    out[2 * q] = -im * k;
+    out[2 * q + 1] = re * k;
+  }
+}
+
+/* force_f(d,:,:,:) = r(nfb:nfe-nfb+1,...)/nfe/nfe/nfe (W/pm.f90:122-124):
+   r is [z][y][x] n^3 (x fastest), out is [z][y][x][3] of m^3 with
+   m = n - 2*lo, taking r from index lo on each axis */
+void oc_force_extract(const float *r, int64_t n, int64_t lo, int64_t m, int64_t d, float *out) {
+  const float fn = (float)n;
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < m; ++z)
+    for (int64_t y = 0; y < m; ++y) {
+      const float *src = r + ((z + lo) * n + (y + lo)) * n + lo;
+      float *dst = out + ((z * m + y) * m) * 3 + d;
+      for (int64_t x = 0; x < m; ++x) {
+        float v = src[x];
+        v = v / fn;
+        v = v / fn;
+        v = v / fn;
+        dst[3 * x] = v;
+      }
+    }
+}
+
+/* max over cells of (F1*F1 + F2*F2) + F3*F3 in real(4) (W/pm.f90:126, :315) */
+float oc_f2_max(const float *f, int64_t ncell) {
+  float mx = 0.0f;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+  for (int64_t q = 0; q < ncell; ++q) {
+    const float a = f[3 * q], b = f[3 * q + 1], c = f[3 * q + 2];
+    const float s = (a * a + b * b) + c * c;
+    if (s > mx) mx = s;
+  }
+  return mx;
+}

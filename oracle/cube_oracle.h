@@ -122,4 +122,8 @@ ocube *oc_alloc(void);
 void oc_free(ocube *c);
 void oc_info(const ocube *c, int64_t *out16);
 void oc_set_npglobal(ocube *c, int64_t n);
+/* FP32 glue of the force solves (W/pm.f90:119-126), threaded */
+void oc_green_mul(const float *crho, const float *kern, float *out, int64_t n);
+void oc_force_extract(const float *r, int64_t n, int64_t lo, int64_t m, int64_t d, float *out);
+float oc_f2_max(const float *f, int64_t ncell);
 #endif

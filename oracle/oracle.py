@@ -76,6 +76,10 @@ def lib():
         L.oc_spine_ptr.restype = P
         L.oc_expansion_n.argtypes = [i64, P, P, f32, f32, f32, P, P]
         L.oc_set_threads.argtypes = [ctypes.c_int]
+        L.oc_green_mul.argtypes = [P, P, P, i64]
+        L.oc_force_extract.argtypes = [P, i64, i64, i64, i64, P]
+        L.oc_f2_max.argtypes = [P, i64]
+        L.oc_f2_max.restype = f32
         L.oc_get_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -162,7 +166,7 @@ def _rfftn(a):
 def _irfftn_unnorm(c, n):
     if c.dtype == np.complex64:
         return sfft.irfftn(c, s=(n, n, n), norm="forward", workers=FFT_WORKERS)
-    return np.fft.irfftn(c, s=(n, n, n), norm="forward")
+    return np.fft.irfftn(c, s=(n, n, n), axes=(0, 1, 2), norm="forward")
 
 
 def kernel_c(nc, ckt, ncell=4, dtype=np.float32, lrck=True, table=True):
@@ -216,6 +220,10 @@ def kernel_c(nc, ckt, ncell=4, dtype=np.float32, lrck=True, table=True):
 def apply_green(crho, kern_d):
     """rho_f(::2)=-crho(2::2)*kern, rho_f(2::2)=crho(::2)*kern (W/pm.f90:119-120)."""
     f = kern_d.dtype.type
+    if f is np.float32 and crho.dtype == np.complex64 and crho.flags.c_contiguous and kern_d.flags.c_contiguous:
+        out = np.empty_like(crho)
+        lib().oc_green_mul(_p(crho), _p(kern_d), _p(out), crho.size)
+        return out
     re = (-crho.imag * kern_d).astype(f)
     im = (crho.real * kern_d).astype(f)
     return (re + 1j * im).astype(np.complex64 if f is np.float32 else np.complex128)
@@ -228,6 +236,11 @@ def fine_force(rho, kern, nfe, nfb):
     crho = _rfftn(rho.astype(f))
     n2 = nfe - 2 * nfb + 2
     out = np.zeros((n2, n2, n2, 3), f)
+    if f is np.float32:
+        for d in range(3):
+            r = np.ascontiguousarray(_irfftn_unnorm(apply_green(crho, kern[d]), nfe), np.float32)
+            lib().oc_force_extract(_p(r), nfe, nfb - 1, n2, d, _p(out))
+        return out, float(lib().oc_f2_max(_p(out), n2 ** 3))
     for d in range(3):
         r = _irfftn_unnorm(apply_green(crho, kern[d]), nfe).astype(f)
         r = r / f(nfe)
@@ -428,3 +441,81 @@ def expansion(a0, dt0, omega_m, omega_l, wde=-1.0):
     da2 = np.zeros_like(a0)
     lib().oc_expansion_n(a0.size, _p(a0), _p(dt0), omega_m, omega_l, wde, _p(da1), _p(da2))
     return da1, da2
+
+
+# --------------------------------------------------------------------------
+# whole-box mesh fields at a chosen precision (the FP64 error statement)
+# --------------------------------------------------------------------------
+def cic_fine(pos, nc, mass, dtype=np.float64):
+    """Fine-mesh CIC of W/pm.f90:50-108 on the whole periodic nf^3 mesh
+    (nf = 4 nc), [z][y][x]: fine coordinate X = ncell*x with x the global
+    coarse position, no half-cell shift (tempx = ncell*(cell-1) +
+    ncell*offset, :63), weights 1-d and d on cells floor(X) and floor(X)+1."""
+    nf = 4 * nc
+    X = 4.0 * np.asarray(pos, np.float64)
+    i0 = np.floor(X).astype(np.int64)
+    d = (X - i0).astype(dtype)
+    return _cic_scatter(i0, d, nf, mass, dtype)
+
+
+def cic_coarse(pos, nc, mass, dtype=np.float64):
+    """Coarse-mesh CIC of W/pm.f90:205-267 on the periodic nc^3 mesh:
+    cell-centred, m = x - 0.5 (tempx = (cell-1) + offset - 0.5, :221)."""
+    m = np.asarray(pos, np.float64) - 0.5
+    i0 = np.floor(m).astype(np.int64)
+    d = (m - i0).astype(dtype)
+    return _cic_scatter(i0, d, nc, mass, dtype)
+
+
+def _cic_scatter(i0, d, n, mass, dtype):
+    rho = np.zeros(n ** 3, dtype)
+    one = dtype(1)
+    for cz in (0, 1):
+        wz = d[:, 2] if cz else one - d[:, 2]
+        for cy in (0, 1):
+            wy = d[:, 1] if cy else one - d[:, 1]
+            for cx in (0, 1):
+                wx = d[:, 0] if cx else one - d[:, 0]
+                idx = (((i0[:, 2] + cz) % n) * n + (i0[:, 1] + cy) % n) * n + (i0[:, 0] + cx) % n
+                rho += np.bincount(idx, weights=(wx * wy * wz).astype(np.float64), minlength=n ** 3).astype(dtype)
+    return (rho * dtype(mass)).reshape(n, n, n)
+
+
+def fine_force_box(rho, fk, pp_on=False, pp_range=1):
+    """Fine force of the whole periodic nf^3 mesh (W/pm.f90:112-124 with the
+    kernel of W/kernel_f.f90 built for nf instead of one tile's nfe): the
+    tile solve equals this inside each tile, the kernel reaching 15 fine
+    cells and the tile buffer 24.  Precision follows rho.dtype; returns
+    [z][y][x][3]."""
+    nf = rho.shape[0]
+    f = rho.dtype.type
+    kern = kernel_f(nf, fk, pp_on, pp_range, dtype=rho.dtype)
+    crho = _rfftn(rho)
+    out = np.zeros(rho.shape + (3,), rho.dtype)
+    for d in range(3):
+        r = _irfftn_unnorm(apply_green(crho, kern[d]), nf).astype(f)
+        out[..., d] = ((r / f(nf)) / f(nf)) / f(nf)
+    return out
+
+
+def fine_potential_box(rho, wg1, pp_on=False, pp_range=1):
+    """Fine potential of the whole periodic nf^3 mesh: the 16^3 table of
+    CUBEnu/kernels/wg1.ascii (README:8-9; wg1 in (i,j,k) order, i fastest)
+    at offsets 0..15, mirrored evenly, its near corner zeroed with PP on
+    like the force kernel (W/kernel_f.f90:29-31), applied in k space:
+    phi = C2R(Re K * R2C(rho)) / nf^3.  Precision follows rho.dtype."""
+    nf = rho.shape[0]
+    f = rho.dtype.type
+    t = np.asarray(wg1, rho.dtype).reshape(16, 16, 16)  # [k][j][i]
+    rs = np.zeros((nf, nf, nf), rho.dtype)
+    rs[:16, :16, :16] = t
+    if pp_on:
+        rs[:pp_range + 1, :pp_range + 1, :pp_range + 1] = 0
+    rs[:, :, nf - 15:] = rs[:, :, 15:0:-1]
+    rs[:, nf - 15:, :] = rs[:, 15:0:-1, :]
+    rs[nf - 15:, :, :] = rs[15:0:-1, :, :]
+    kre = _rfftn(rs).real.astype(f)
+    crho = _rfftn(rho)
+    prod = (crho.real * kre).astype(f) + 1j * (crho.imag * kre).astype(f)
+    r = _irfftn_unnorm(prod.astype(crho.dtype), nf).astype(f)
+    return ((r / f(nf)) / f(nf)) / f(nf)

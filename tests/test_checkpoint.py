@@ -48,3 +48,15 @@ def test_write_refuses_codes_wider_than_the_header(tmp_path):
     with pytest.raises(ValueError):
         ck.write(str(tmp_path), 0.0, h, xp, np.zeros((4, 3), np.int16), np.zeros(8, np.int32),
                  np.zeros((8, 3), np.float32))
+
+
+def test_config1_fixture_initial_state_digest():
+    """The full-run fixture of the 128^3-particle configuration starts from
+    the state its recipe rebuilds here, byte for byte (the GPU test relies on
+    rebuilding the same initial state)."""
+    from tests.golden import make_config1_fullrun as gen
+    fx = np.load(gen.OUT)
+    assert gen.digest(gen.initial_state()) == str(fx["ic_digest"])
+    s = fx["schedule"]
+    assert s.shape[1] == 3 and abs(float(fx["a_final"]) - 1.0) < 1e-5
+    assert np.all(np.diff(s[:, 2]) > 0)

@@ -4,7 +4,7 @@ longer fits a float exactly and the top code of a fine sub-cell rounds onto
 the next sub-cell's boundary.  The deposit's candidate filters use the
 codes' top bits and so skip such a particle in the cell above, which is
 exact only because its weight past the boundary is then exactly zero.
-nc=132 in one tile, particles packed into the cells 124..131 (and a few
+nc=144 in one tile, particles packed into the cells 136..143 (and a few
 elsewhere) with most codes at sub-cell tops; the whole fine mesh must match
 the oracle bit for bit."""
 import numpy as np
@@ -16,13 +16,13 @@ from tests.helpers import global_from_tiles
 
 pytestmark = pytest.mark.gpu
 
-NC, NNT = 132, 1
+NC, NNT = 144, 1
 
 
 def _state():
     rng = np.random.default_rng(5)
     tops = np.array([16383, 32767, 49151, 65535])
-    cells = np.stack(np.meshgrid(*[np.arange(124, 132)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    cells = np.stack(np.meshgrid(*[np.arange(NC - 8, NC)] * 3, indexing="ij"), -1).reshape(-1, 3)
     cells = np.concatenate([cells, rng.integers(0, NC, size=(300, 3))])
     c = np.repeat(cells, 5, axis=0)
     u = np.where(rng.random(c.shape) < 0.7, tops[rng.integers(0, 4, c.shape)], rng.integers(0, 65536, c.shape))

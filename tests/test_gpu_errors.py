@@ -131,7 +131,7 @@ def test_cell_above_lds_sort_capacity_bit_exact():
     oc = O.OracleCube(nc, nnt)
     oc.state_from_xv(pos, vel.astype(np.float32).astype(np.float64), None, sig)
     so = oc.get_state()
-    assert so["rhoc"].max() == n_dense
+    assert so["rhoc"].max() >= n_dense  # the dense cell (and the few uniform particles that fall in it)
     sg = g.get_state()
     for k in ("rhoc", "xp", "vp"):
         assert np.array_equal(sg[k], so[k]), k

@@ -153,13 +153,17 @@ def test_kicks_bit_exact_given_forces(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_ids)
-@pytest.mark.parametrize("stage_max", [None, "0", "40"])
-def test_pp_bit_exact(case, stage_max, monkeypatch):
+@pytest.mark.parametrize("stage_max,big_max", [(None, None), ("0", None), ("40", "0"), ("0", "0"), ("40", "200")])
+def test_pp_bit_exact(case, stage_max, big_max, monkeypatch):
     """PP stages each 4^3-cell block's neighbourhood in LDS; blocks with more
-    than CUBE_PP_STAGE_MAX particles there walk the global lists.  Both walks
-    (all staged, none staged, mixed) must match the oracle bit for bit."""
+    than CUBE_PP_STAGE_MAX particles there go to a second launch with a
+    larger staging area (CUBE_PP_BIG_MAX), and blocks beyond that walk the
+    global lists.  Every mix of the three paths must match the oracle bit
+    for bit."""
     if stage_max is not None:
         monkeypatch.setenv("CUBE_PP_STAGE_MAX", stage_max)
+    if big_max is not None:
+        monkeypatch.setenv("CUBE_PP_BIG_MAX", big_max)
     st, oc, g = _pair(case, pp_on=True)
     m = st["mass_p"]
     f_o = oc.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
@@ -169,13 +173,16 @@ def test_pp_bit_exact(case, stage_max, monkeypatch):
 
 
 @pytest.mark.parametrize("pp_range", [0, 2])
-@pytest.mark.parametrize("stage_max", [None, "0"])
+@pytest.mark.parametrize("stage_max", [None, "0", "00"])
 def test_pp_other_ranges_bit_exact(pp_range, stage_max, monkeypatch):
     """pp_range 0 and 2: 1 and 25 (kk, jj) rows of 1 and 5 fine cells per
-    particle, staged and global walks."""
+    particle; staged, deferred to the large-LDS launch ("0"), and global
+    walks ("00": both caps zero)."""
     from cafproject_amd.cube import Cube
     if stage_max is not None:
         monkeypatch.setenv("CUBE_PP_STAGE_MAX", stage_max)
+        if stage_max == "00":
+            monkeypatch.setenv("CUBE_PP_BIG_MAX", "0")
     nc, nnt = 24, 2
     st = zeldovich_state(nc, nnt, seed=43)
     oc = O.OracleCube(nc, nnt, pp_on=True, pp_range=pp_range)

@@ -164,3 +164,28 @@ def test_pp_force_against_brute_force():
     q = np.abs(vt[np.clip(code + 1, 0, 65535)] - vt[np.clip(code - 1, 0, 65535)])
     assert np.all(np.abs(dv - want) <= q + 1e-12)
     assert abs(f2 / float(np.float32((F * F).sum(1).max())) - 1) < 1e-6
+
+
+def test_threaded_force_glue_equals_numpy():
+    """The oracle's threaded C glue of the FP32 fine force (Green multiply,
+    /nfe three times, extraction, max |F|^2) gives the same bits as the
+    element-wise numpy expressions of W/pm.f90:119-126."""
+    import scipy.fft as sfft
+    from oracle import oracle as O
+    rng = np.random.default_rng(3)
+    nfe, nfb = 40, 8
+    rho = rng.standard_normal((nfe, nfe, nfe)).astype(np.float32)
+    kern = rng.standard_normal((3, nfe, nfe, nfe // 2 + 1)).astype(np.float32)
+    out, f2 = O.fine_force(rho, kern, nfe, nfb)
+    f = np.float32
+    crho = sfft.rfftn(rho)
+    n2 = nfe - 2 * nfb + 2
+    ref = np.zeros((n2, n2, n2, 3), f)
+    for d in range(3):
+        g = ((-crho.imag * kern[d]).astype(f) + 1j * (crho.real * kern[d]).astype(f)).astype(np.complex64)
+        r = sfft.irfftn(g, s=(nfe,) * 3, norm="forward").astype(f)
+        r = r / f(nfe); r = r / f(nfe); r = r / f(nfe)
+        ref[..., d] = r[nfb - 1:nfe - nfb + 1, nfb - 1:nfe - nfb + 1, nfb - 1:nfe - nfb + 1]
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    f2r = ((ref[..., 0] * ref[..., 0] + ref[..., 1] * ref[..., 1]) + ref[..., 2] * ref[..., 2]).max()
+    assert np.float32(f2) == f2r
