@@ -232,7 +232,8 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
     delete c;
     return CUBE_ERR_ALLOC;
   }
-  if ((st = cube_fft_create(c, (int)c->nf, &c->plan_f)) || (st = cube_fft_create(c, (int)c->nc, &c->plan_c))) {
+  if ((st = cube_fft_create(c, (int)c->nf, &c->plan_f)) || (st = cube_fft_create(c, (int)c->nc, &c->plan_c)) ||
+      (st = cube_power_scratch_setup(c->device))) {
     free_all(c);
     delete c;
     return st;

@@ -284,6 +284,9 @@ int cube_tile_boxsum(cube_ctx *c, const int32_t *rhoc, int w, int64_t *host_out)
 
 /* 1 when the line-FFT kernels are built for size n (12..2048, factors 2, 3, 5) */
 int cube_fft_supported(int64_t n);
+/* cube_power_noalloc's device scratch: address and twiddles, once per
+   device, with that device current (cube_ctx_create calls it) */
+int cube_power_scratch_setup(int device);
 /* the n twiddles exp(-2 pi i m/n) every plan of size n holds (host) */
 void cube_fft_twiddles(int n, float2 *h);
 int cube_fft_create(cube_ctx *c, int n, FFTPlan **out);

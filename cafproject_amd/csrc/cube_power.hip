@@ -8,6 +8,7 @@
 #include "cube_internal.h"
 #include <math.h>
 #include <string.h>
+#include <mutex>
 
 #define PW_T 256
 
@@ -186,5 +187,75 @@ extern "C" int cube_power(cube_ctx *c, int32_t ng, int32_t nbin, double *ksum, d
   }
   (void)hipStreamSynchronize(c->stream);
   cube_fft_destroy(plan);
+  return st;
+}
+
+/* ---------------- cube_power_noalloc ----------------
+ * The same spectrum without any allocation in the call: the twiddle tables
+ * of every built FFT size and the bin sums live in one file-scope device
+ * array.  Host code never passes the array's name to a kernel or a copy
+ * (on the host it is a stand-in, not a device address): the address of
+ * each device's copy is taken once, at context creation on that device, and
+ * kept per device.  Calls that overlap on one device share the array and
+ * race; each call clears the bins it uses before the deposit. */
+#define PW_TW_MAX 5632   /* float2 twiddles of all built sizes (5362) */
+#define PW_NBIN_MAX 1774 /* nint(2048/2 * sqrt(3)): bins of the largest mesh */
+#define PW_SCRATCH_DOUBLES (PW_TW_MAX + 3 * (PW_NBIN_MAX + 1))
+#define PW_MAX_DEV 64
+
+static __device__ __align__(256) double scratch[PW_SCRATCH_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch[PW_MAX_DEV];  /* device address of scratch, per device */
+static int g_tw_off[2049];             /* float2 offset of size n's twiddles */
+
+/* Once per device, with the device current: find scratch's address and
+   write the twiddles of every built size into it. */
+int cube_power_scratch_setup(int device) {
+  if (device < 0 || device >= PW_MAX_DEV) return CUBE_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return CUBE_OK;
+  double *ptr = nullptr;
+  if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return CUBE_ERR_HIP;
+  float2 h[2048];
+  int off = 0;
+  for (int n = 1; n <= 2048; ++n) {
+    if (!cube_fft_supported(n)) continue;
+    if (off + n > PW_TW_MAX) return CUBE_ERR_INVALID;
+    cube_fft_twiddles(n, h);
+    if (hipMemcpy(reinterpret_cast<float2 *>(ptr) + off, h, n * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+      return CUBE_ERR_HIP;
+    g_tw_off[n] = off;
+    off += n;
+  }
+  g_scratch[device] = ptr;
+  return CUBE_OK;
+}
+
+extern "C" int cube_power_noalloc(cube_ctx *c, int32_t ng, int32_t nbin, double *ksum, double *psum,
+                                  int64_t *count) {
+  if (c) c->pm_ahead = 0;
+  int st = power_args(c, ng, nbin, ksum, psum, count);
+  if (st) return st;
+  if (nbin > PW_NBIN_MAX || ng > 2048) return CUBE_ERR_INVALID; /* beyond the scratch array */
+  double *ptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ptr = (c->device >= 0 && c->device < PW_MAX_DEV) ? g_scratch[c->device] : nullptr;
+  }
+  if (!ptr) return CUBE_ERR_STATE; /* set up by cube_ctx_create on this device */
+  CUBE_CHECK(hipSetDevice(c->device));
+  float2 *tw = reinterpret_cast<float2 *>(ptr) + g_tw_off[ng];
+  double *bins = ptr + PW_TW_MAX;
+  st = power_device(c, ng, nbin, tw, bins);
+  /* bins 1..nbin straight into the caller's arrays (the counts are
+     unsigned 64-bit, the same bits as int64 below 2^63) */
+  if (!st && (hipMemcpyAsync(ksum, bins + 1, nbin * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(psum, bins + (nbin + 1) + 1, nbin * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream) != hipSuccess ||
+              hipMemcpyAsync(count, bins + 2 * (nbin + 1) + 1, nbin * sizeof(int64_t), hipMemcpyDeviceToHost,
+                             c->stream) != hipSuccess))
+    st = CUBE_ERR_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !st) st = CUBE_ERR_HIP;
   return st;
 }

@@ -80,7 +80,7 @@ EXPORTS = ["cube_ctx_create", "cube_ctx_destroy", "cube_strerror", "cube_ctx_inf
            "cube_pm_fine_force", "cube_pm_fine_kick", "cube_pm_coarse_deposit", "cube_pm_coarse_force",
            "cube_pm_coarse_kick", "cube_get_field", "cube_set_field", "cube_sync", "cube_host_atan",
            "cube_profile", "cube_timers", "cube_ic_zeldovich", "cube_set_state_xv", "cube_power",
-           "cube_pm_fine_potential"]
+           "cube_power_noalloc", "cube_pm_fine_potential"]
 TIMERS = ["drift", "vstats", "pp", "fine_deposit", "fine_fft", "fine_green", "fine_invy", "fine_c2r",
           "fine_kick", "coarse_deposit", "coarse_force", "coarse_kick", "checks"]
 
@@ -115,6 +115,7 @@ def lib():
             getattr(L, n).argtypes = [P]
         L.cube_pp_force.argtypes = [P, f32, f32, f32, f32, P]
         L.cube_power.argtypes = [P, ctypes.c_int32, ctypes.c_int32, P, P, P]
+        L.cube_power_noalloc.argtypes = [P, ctypes.c_int32, ctypes.c_int32, P, P, P]
         L.cube_particle_mesh.argtypes = [P, f32, f32, f32, f32, f32, ctypes.POINTER(PMOut)]
         L.cube_pm_fine_deposit.argtypes = [P, f32, P]
         L.cube_pm_fine_force.argtypes = [P, P]
@@ -372,13 +373,15 @@ class Cube:
     def sync(self):
         self._check(lib().cube_sync(self._h), "sync")
 
-    def power(self, ng, box):
-        """P(k) of the current state on an ng^3 mesh (cube_power): returns
-        (k in h/Mpc, Delta^2, modes) per linear bin 1..nint(ng/2 sqrt 3),
-        as cafproject_amd.power.state_power does on the host."""
+    def power(self, ng, box, noalloc=False):
+        """P(k) of the current state on an ng^3 mesh (cube_power, or
+        cube_power_noalloc with noalloc=True): returns (k in h/Mpc, Delta^2,
+        modes) per linear bin 1..nint(ng/2 sqrt 3), as
+        cafproject_amd.power.state_power does on the host."""
         nbin = int(round(ng // 2 * np.sqrt(3.0)))
         ks, ps = np.zeros(nbin), np.zeros(nbin)
         cnt = np.zeros(nbin, np.int64)
-        self._check(lib().cube_power(self._h, ng, nbin, _ptr(ks), _ptr(ps), _ptr(cnt)), "power")
+        fn = lib().cube_power_noalloc if noalloc else lib().cube_power
+        self._check(fn(self._h, ng, nbin, _ptr(ks), _ptr(ps), _ptr(cnt)), "power")
         with np.errstate(invalid="ignore", divide="ignore"):
             return ks / cnt * 2 * np.pi / box, ps / cnt, cnt

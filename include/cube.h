@@ -232,6 +232,16 @@ int cube_sync(cube_ctx *ctx);
    and the mode counts; the caller divides.  Uses the fine-mesh buffers. */
 int cube_power(cube_ctx *ctx, int32_t ng, int32_t nbin, double *ksum, double *psum, int64_t *count);
 
+/* cube_power without any allocation in the call.  No counterpart in the
+   reference: for callers that cannot allocate device memory (code under
+   HIP graph capture, in-loop validation of a serving loop).  The twiddle
+   tables and the bin sums live in a fixed device array of the library
+   (about 88 KB per device, held until the process ends); meshes up to
+   ng = 2048 and nbin <= 1774, else CUBE_ERR_INVALID.  Calls that overlap on
+   one device share that array and race.  Same arguments and results as
+   cube_power (its FP64 sums are atomic, so equal to rounding). */
+int cube_power_noalloc(cube_ctx *ctx, int32_t ng, int32_t nbin, double *ksum, double *psum, int64_t *count);
+
 /* Host-side twins of device math, exported for the bindings' checks. */
 double cube_host_atan(double x);
 
