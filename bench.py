@@ -299,7 +299,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-gradient ensemble measurement")
     ap.add_argument("--batched-iters", type=int, default=3)
-    ap.add_argument("--cpu-reps", type=int, default=1, help="oracle steps timed for the CPU baseline (median)")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="oracle steps timed for the CPU baseline (median)")
     ap.add_argument("--no-large", action="store_true", help="skip the 1024^3-particle large-box measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy kernel (torch), the reference "

@@ -187,6 +187,7 @@ struct DriftP {
   int nvh;        /* nvbin/2 */
   float dt_mid;
   double xres;
+  double inv_xres4; /* 1/(xres*ncell), a power of two: multiplying by it is the division, exactly */
   int agg_cap;    /* largest row (particles) whose slots are aggregated in LDS */
 };
 #define AGG_CAP 4096
@@ -577,11 +578,13 @@ __global__ __launch_bounds__(256) void k_drift_final2(
     for (int a = 0; a < 3; ++a) v[a] = vdec[(int32_t)vp[3 * p + a] + P.nvh] + (double)vof[a];
   };
 
-  float acc[3] = {0.f, 0.f, 0.f};
-  if (tid < ncl) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) acc[a] = (float)((double)vfield_old[3 * (c0 + tid) + a] * WEIGHT_V);
-  }
+  /* vfield_new: thread (component aa, cell ca) sums its cell's slots in
+     slot order -- three independent serial chains per cell, so a crowded
+     cell holds its workgroup a third as long */
+  const int ca = tid & 63, aa = tid >> 6;
+  const bool accum = aa < 3 && ca < ncl;
+  float acc = 0.f;
+  if (accum) acc = (float)((double)vfield_old[3 * (c0 + ca) + aa] * WEIGHT_V);
   for (int64_t ch = S0; ch < S1; ch += FB_CH) {
     const int64_t ce = min(ch + FB_CH, S1);
     for (int64_t s = ch + tid; s < ce; s += 256) {
@@ -595,27 +598,23 @@ __global__ __launch_bounds__(256) void k_drift_final2(
       pix[s - ch] = (uint64_t)p;
     }
     __syncthreads();
-    if (tid < ncl) {
-      const int64_t lo = max(cst[tid], ch), hi = min(cst[tid + 1], ce);
-      for (int64_t q = lo; q < hi; ++q) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) acc[a] = (float)((double)acc[a] + vr[q - ch][a]);
-      }
+    if (accum) {
+      const int64_t lo = max(cst[ca], ch), hi = min(cst[ca + 1], ce);
+      for (int64_t q = lo; q < hi; ++q) acc = (float)((double)acc + vr[q - ch][aa]);
     }
     __syncthreads();
   }
+  if (accum) {
+    const double n = (double)(cst[ca + 1] - cst[ca]);
+    vn[ca][aa] = (float)((double)acc / (n + WEIGHT_V));
+    vfield_new[3 * (c0 + ca) + aa] = vn[ca][aa];
+  }
+  __syncthreads();
   double sc = 0, sr = 0, ss = 0;
   if (tid < ncl) {
-    const double n = (double)(cst[tid + 1] - cst[tid]);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      vn[tid][a] = (float)((double)acc[a] / (n + WEIGHT_V));
-      vfield_new[3 * (c0 + tid) + a] = vn[tid][a];
-    }
     const float v0 = vn[tid][0], v1 = vn[tid][1], v2 = vn[tid][2];
     sc = (double)((v0 * v0 + v1 * v1) + v2 * v2);
   }
-  __syncthreads();
   for (int64_t s = S0 + tid; s < S1; s += 256) {
     int lc;
     int64_t p;
@@ -633,7 +632,7 @@ __global__ __launch_bounds__(256) void k_drift_final2(
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       /* xp_new=xp+nint(dt_mid*vreal/(x_resolution*ncell)) (:115) */
-      const double qx = ((double)P.dt_mid * v[a]) / (P.xres * 4.0);
+      const double qx = ((double)P.dt_mid * v[a]) * P.inv_xres4;
       xp2[3 * s + a] = (typename IntOf<IX>::T)cube_wrap(IX, (int64_t)xp[3 * p + a] + (int64_t)cube_nint(qx));
       /* vp_new=nint(...atan(...*(vreal-vfield_new))...) (:116-117) */
       const int32_t code = cube_encode_v(IV, venc, v[a] - (double)vn[lc][a]);
@@ -868,6 +867,7 @@ int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi
   P.nvh = c->nvbin / 2;
   P.dt_mid = dt_mid;
   P.xres = c->p.izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0;
+  P.inv_xres4 = c->p.izipx == 1 ? 64.0 : 16384.0;
   P.agg_cap = AGG_CAP;
   /* a lower cap sends more rows down the all-global slot path (used by the
      tests to cover both paths in one run) */

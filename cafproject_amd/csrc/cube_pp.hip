@@ -27,6 +27,26 @@ __device__ static inline double powi5(double x) {
   const double x2 = x * x;
   return x2 * (x * x2);
 }
+/* x[a] / r3 for the three components, IEEE quotients either way.  With
+   one_div a single division forms y = RN(1/r3) and each quotient is
+   q0 = RN(x*y) corrected as RN(q0 + (x - r3*q0)*y) by two fused
+   multiply-adds: for normal operands (here |x| < 64, r3 > 1e-3) that is
+   the correctly rounded x/r3 (Markstein: y within half an ulp of 1/r3,
+   q0 within one ulp of x/r3, the residual exact), bit for bit the
+   division. */
+__device__ static inline void quot3(const double x[3], double r3, double q[3], bool one_div) {
+  if (one_div) {
+    const double y = 1.0 / r3;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double q0 = x[a] * y;
+      q[a] = fma(fma(-r3, q0, x[a]), y, q0);
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q[a] = x[a] / r3;
+  }
+}
 
 template <int IX, int IV>
 __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restrict__ xp,
@@ -34,7 +54,7 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
                                            const int32_t *__restrict__ rhoc, const int64_t *__restrict__ cstart,
                                            const double *__restrict__ vdec, Geo g, int nvh, int pr, double rsoft,
                                            double mass, double a_mid, double dt, double venc,
-                                           unsigned long long *__restrict__ out) {
+                                           unsigned long long *__restrict__ out, int fdiv) {
   const int nt = g.nt;
   const int64_t row = blockIdx.x;
   const int j = (int)(row % nt), k = (int)((row / nt) % nt), t = (int)(row / ((int64_t)nt * nt));
@@ -93,9 +113,11 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
             const double rcut = rmag / 16.0;
             const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
             const double r3 = powi3(rmag);
+            double qt[3];
+            quot3(x21, r3, qt, fdiv);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-              double fp = (mass * (x21[a] / r3)) * pcut;
+              double fp = (mass * qt[a]) * pcut;
               if (!(rmag > rsoft)) fp = fp * 0.0;
               ft[a] = ft[a] + fp;
             }
@@ -167,7 +189,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
                                                   const int64_t *__restrict__ cstart,
                                                   const double *__restrict__ vdec, const double *__restrict__ vtab, Geo g, int nvh, int pr,
                                                   double rsoft, double mass, double a_mid, double dt, double venc,
-                                                  int maxs, unsigned long long *__restrict__ out,
+                                                  int maxs, int fdiv, unsigned long long *__restrict__ out,
                                                   int32_t *__restrict__ dlist, unsigned long long *__restrict__ dcount,
                                                   const int32_t *__restrict__ list,
                                                   const unsigned long long *__restrict__ nlist) {
@@ -358,9 +380,11 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
         const double rcut = rmag / 16.0;
         const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
         const double r3 = powi3(rmag);
+        double qt[3];
+        quot3(x21, r3, qt, fdiv);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          double fp = (mass * (x21[a] / r3)) * pcut;
+          double fp = (mass * qt[a]) * pcut;
           if (!(rmag > rsoft)) fp = fp * 0.0;
           ft[a] = ft[a] + fp;
         }
@@ -399,9 +423,11 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
               const double rcut = rmag / 16.0;
               const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
               const double r3 = powi3(rmag);
+              double qt[3];
+              quot3(x21, r3, qt, fdiv);
 #pragma unroll
               for (int a = 0; a < 3; ++a) {
-                double fp = (mass * (x21[a] / r3)) * pcut;
+                double fp = (mass * qt[a]) * pcut;
                 if (!(rmag > rsoft)) fp = fp * 0.0;
                 ft[a] = ft[a] + fp;
               }
@@ -452,6 +478,10 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 template <int IX, int IV>
 static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mass_p, unsigned long long *out) {
   const int pr = (int)c->p.pp_range;
+  /* one division per pair (quot3); CUBE_PP_FASTDIV=0 takes the three
+     divisions instead (the same bits) */
+  const char *fd = getenv("CUBE_PP_FASTDIV");
+  const int fdiv = !(fd && atoi(fd) == 0);
   if (c->nt % 4 == 0 && pr >= 0 && pr <= 4) {
     const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
     const size_t table = (((size_t)nreg + 1) * 2 + 7) / 8 * 8; /* uint16 per fine cell, plus cur[-1] */
@@ -477,7 +507,7 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
     hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3((unsigned)nblk), dim3(PPB_T), (size_t)maxs * 8 + table, c->stream,
                        (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
                        c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p, (double)a_mid,
-                       (double)dt, cube_vconst(sigma_vi), maxs, out, two ? c->ppdefer : nullptr, dcount, nullptr,
+                       (double)dt, cube_vconst(sigma_vi), maxs, fdiv, out, two ? c->ppdefer : nullptr, dcount, nullptr,
                        nullptr);
     CUBE_LAUNCH_CHECK();
     if (two) {
@@ -489,7 +519,7 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
       hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3(grid), dim3(PPB_T), (size_t)maxb * 8 + table, c->stream,
                          (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
                          c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p,
-                         (double)a_mid, (double)dt, cube_vconst(sigma_vi), maxb, out, nullptr, nullptr,
+                         (double)a_mid, (double)dt, cube_vconst(sigma_vi), maxb, fdiv, out, nullptr, nullptr,
                          c->ppdefer, dcount);
       CUBE_LAUNCH_CHECK();
     }
@@ -499,7 +529,7 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
   hipLaunchKernelGGL((k_pp<IX, IV>), dim3((unsigned)nrows), dim3(PT), 0, c->stream,
                      (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
                      c->vdec0, geo_of(c), c->nvbin / 2, (int)c->p.pp_range, (double)c->p.rsoft, (double)mass_p,
-                     (double)a_mid, (double)dt, cube_vconst(sigma_vi), out);
+                     (double)a_mid, (double)dt, cube_vconst(sigma_vi), out, fdiv);
   CUBE_LAUNCH_CHECK();
   return CUBE_OK;
 }

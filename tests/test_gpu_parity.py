@@ -172,6 +172,20 @@ def test_pp_bit_exact(case, stage_max, big_max, monkeypatch):
     assert f_g == f_o
 
 
+@pytest.mark.parametrize("case", CASES[:2], ids=_ids)
+def test_pp_three_divisions_bit_exact(case, monkeypatch):
+    """The pair force's three IEEE divisions x/r^3 (CUBE_PP_FASTDIV=0) and
+    the default single division with corrected products give the oracle's
+    bits alike."""
+    monkeypatch.setenv("CUBE_PP_FASTDIV", "0")
+    st, oc, g = _pair(case, pp_on=True)
+    m = st["mass_p"]
+    f_o = oc.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
+    f_g = g.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
+    _same_state(g.get_state(), oc.get_state())
+    assert f_g == f_o
+
+
 @pytest.mark.parametrize("pp_range", [0, 2])
 @pytest.mark.parametrize("stage_max", [None, "0", "00"])
 def test_pp_other_ranges_bit_exact(pp_range, stage_max, monkeypatch):
