@@ -286,6 +286,71 @@ def batched_gradients(iters=3):
             "loss_finite": bool(np.all(np.isfinite(loss)))}
 
 
+Z_BINS = (49.0, 20.0, 10.0, 5.0, 3.0, 2.0, 1.0, 0.5, 0.25, 0.0)
+
+
+def to_z0(nc, nnt, box, izipx, pp_on, max_steps=3000):
+    """The whole run of the headline workload, z=49 -> 0 with the
+    reference's step criterion (W/main.f90:88-145): every step's wall time
+    and stage times (PP, deposits, drift, kicks) binned by redshift, the
+    particle count after every step, and P(k) of the final state."""
+    from cafproject_amd.cube import Cube
+    g = Cube(nc, nnt, izipx=izipx, izipv=2, has_pid=False, pp_on=pp_on, pp_range=1, rsoft=0.3)
+    g.init_kernels()
+    st = make_state_gpu(g, box, 49.0, 42, particles=False)
+    n = st["np"]
+    del st["host"]
+    mass = float(np.float32((4 * nc) ** 3 / n))
+    step = Step(g, st, mass)
+    g.profile(True)
+    g.timers(reset=True)
+    rows = []
+    t0 = time.perf_counter()
+    while not step.clock.final_step and len(rows) < max_steps:
+        a0 = float(step.clock.a)
+        t = time.perf_counter()
+        step()
+        g.sync()
+        ms = 1e3 * (time.perf_counter() - t)
+        tm = {k: v[0] / max(v[1], 1) for k, v in g.timers(reset=True).items()}
+        if g.nplocal != n:
+            raise RuntimeError(f"particle count {g.nplocal} != {n} after step {len(rows) + 1}")
+        a1 = float(step.clock.a)
+        rows.append(dict(z=1 / a1 - 1, ms=ms, pp=tm.get("pp", 0.0), drift=tm["drift"],
+                         fine_deposit=tm["fine_deposit"], coarse_deposit=tm["coarse_deposit"],
+                         kick=tm["fine_kick"] + tm["coarse_kick"], da=a1 - a0))
+        if len(rows) % 10 == 0 or step.clock.final_step:
+            log(f"[bench] z0 run step {len(rows)} a={a1:.4f} z={1 / a1 - 1:.3f} {ms:.1f} ms "
+                f"(pp {rows[-1]['pp']:.1f}, deposits {tm['fine_deposit']:.1f}/{tm['coarse_deposit']:.1f})")
+    total = time.perf_counter() - t0
+    g.profile(False)
+    bins = []
+    for zhi, zlo in zip(Z_BINS[:-1], Z_BINS[1:]):
+        sel = [r for r in rows if zlo <= r["z"] < zhi or (zhi == Z_BINS[0] and r["z"] >= zhi)]
+        if not sel:
+            continue
+        mean = {k: round(sum(r[k] for r in sel) / len(sel), 2)
+                for k in ("ms", "pp", "drift", "fine_deposit", "coarse_deposit", "kick")}
+        bins.append(dict(z_range=[zhi, zlo], steps=len(sel), ms_per_step=mean["ms"],
+                         max_ms=round(max(r["ms"] for r in sel), 2),
+                         stage_ms={k: v for k, v in mean.items() if k != "ms"},
+                         pp_max_ms=round(max(r["pp"] for r in sel), 2)))
+    k, d2, cnt = g.power(2 * nc, box)
+    pk = {f"{kk:.3f}": float(d2[int(np.argmin(np.abs(k - kk)))]) for kk in (0.05, 0.1, 0.3, 1.0)}
+    del step, g
+    return {
+        "metric": "wall seconds of the run z=49 -> 0 (512^3 PM+PP)", "value": total, "unit": "s",
+        "n_gpus": 1, "steps": len(rows), "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "ms_per_step": 1e3 * total / max(len(rows), 1), "vs_baseline": None,
+        "config": {"workload": f"CUBE PM+PP run z=49 -> 0, {round(n ** (1 / 3))}^3 particles, {box:g} Mpc/h, "
+                               f"nc={nc}, {nnt}^3 tiles, int{8 * izipx} x / int16 v, PP {'on' if pp_on else 'off'}, "
+                               f"Zel'dovich EH no-wiggle, seed 42", "particles": n},
+        "a_final": 1 / (1 + rows[-1]["z"]) if rows else None,
+        "particles_conserved": True, "by_redshift": bins,
+        "delta2_z0": {"k_h_per_mpc": list(pk), "delta2": list(pk.values())},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,11 +369,16 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy kernel (torch), the reference "
                          "point for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--to-z0", action="store_true",
+                    help="instead of the timed steps: the whole run z=49 -> 0, step times by redshift")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU (the whole box on one device); "
               f"--gpus {args.gpus} is not supported. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.to_z0:
+        print(json.dumps(to_z0(args.nc, args.nnt, args.box, args.izipx, not args.no_pp)), flush=True)
+        return
     from cafproject_amd.cube import Cube
     if args.calibrate_copy:
         import torch

@@ -63,7 +63,7 @@ static void free_all(cube_ctx *c) {
                   c->cstart, c->cstart2, c->dest, c->key, c->sortbuf, c->fill, c->xdest, c->xkey, c->rank, c->xrank, c->fbflag,
                   c->tanf_tab, c->venc_tab, c->vdec0, c->vdec1, c->rho_f, c->spec_f, c->specd_f, c->force_f,
                   c->kern_f, c->rho_c, c->spec_c, c->specd_c, c->force_c, c->kern_c, c->dred,
-                  c->ured, c->ired, c->scanbuf, c->vpart, c->slots, c->ppdefer};
+                  c->ured, c->ired, c->scanbuf, c->vpart, c->slots, c->ppdefer, c->ppitems};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   cube_fft_destroy(c->plan_f);
@@ -182,6 +182,7 @@ extern "C" int cube_ctx_create(const cube_params *p, cube_ctx **out) {
   st |= dalloc(&c->cstart2, c->ncells + 1);
   st |= dalloc(&c->fill, c->ncells);
   st |= dalloc(&c->ppdefer, c->ncells / 64 + 1);
+  st |= dalloc(&c->ppitems, c->ncells / 64 + 2);
   st |= dalloc(&c->fbflag, c->ncells / 64 + 2); /* one per 64-cell finalize workgroup of the drift */
   st |= dalloc(&c->tanf_tab, c->nvbin);
   st |= dalloc(&c->venc_tab, c->nvbin / 2);

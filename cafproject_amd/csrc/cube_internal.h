@@ -76,7 +76,8 @@ struct cube_ctx {
   float2 *specd_f;          /* fine_nspec * pitch*nf*nf */
   int fine_nspec;           /* force spectra held: 3, or 1 in the compact layout */
   int scratch_in_force;     /* the drift's sort scratch lives inside force_f */
-  int32_t *ppdefer;         /* ncells/64: PP blocks deferred to the large-LDS launch */
+  int32_t *ppdefer;         /* ncells/64: PP blocks deferred to the cell path */
+  int32_t *ppitems;         /* ncells/64 + 1: their work items, then prefix sums */
   float *force_f;           /* 3*nf^3, (d fastest) */
   float *kern_f;            /* octant in 16-column blocks, cube_kernf_index */
   /* coarse mesh */

@@ -48,6 +48,24 @@ __device__ static inline void quot3(const double x[3], double r3, double q[3], b
   }
 }
 
+/* the pair force of a neighbour at x21 (W/ext_pp_force.f90:100-115), added
+   to ft; every PP path forms it with this code */
+__device__ __forceinline__ void pp_add(const double x21[3], double rsoft, double mass, int fdiv, double ft[3]) {
+  double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
+  if (rmag == 0.0) rmag = 1.0;
+  const double rcut = rmag / 16.0;
+  const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
+  const double r3 = powi3(rmag);
+  double qt[3];
+  quot3(x21, r3, qt, fdiv);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double fp = (mass * qt[a]) * pcut;
+    if (!(rmag > rsoft)) fp = fp * 0.0;
+    ft[a] = ft[a] + fp;
+  }
+}
+
 template <int IX, int IV>
 __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restrict__ xp,
                                            typename IntOf<IV>::T *__restrict__ vp,
@@ -108,19 +126,7 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
             double x21[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) x21[a] = ((double)(4 * (cc[a] - 1)) + 4.0 * cube_xoff(IX, cd[a])) - x1[a];
-            double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
-            if (rmag == 0.0) rmag = 1.0;
-            const double rcut = rmag / 16.0;
-            const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
-            const double r3 = powi3(rmag);
-            double qt[3];
-            quot3(x21, r3, qt, fdiv);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              double fp = (mass * qt[a]) * pcut;
-              if (!(rmag > rsoft)) fp = fp * 0.0;
-              ft[a] = ft[a] + fp;
-            }
+            pp_add(x21, rsoft, mass, fdiv, ft);
           }
         }
 #pragma unroll
@@ -160,26 +166,69 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
  * relative to the region's corner (under 2^20 per axis) and packed 21 bits
  * per axis into one 64-bit word.  x2 - x1 = (X2 - X1) * 2^-15 is then the
  * same FP64 value the reference forms, and each particle sums its pairs in
- * the same order, so the result is bit-identical to the unstaged walk.
+ * the same order, so the result is bit-identical to the reference's walk.
  *
  * A fine cell lies in exactly one coarse cell, so one thread per staged
  * coarse cell counts its particles into the per-fine-cell table and, after
  * the scan, places them walking l downwards, with plain LDS updates.
  *
- * Two launches share the kernel.  The first, one workgroup per block with
- * a small staging area (several workgroups per CU), defers a block whose
- * neighbourhood holds more than `maxs` particles -- a clustered region --
- * to the list `dlist`.  The second walks that list with a staging area of
- * most of a CU's LDS (one workgroup per CU, `list`/`nlist`); a block that
- * overflows even that walks the global lists instead.  Every block's
- * particles are kicked exactly once, by the same arithmetic either way.
+ * A block whose neighbourhood holds more than `maxs` particles -- a
+ * clustered region -- is left to the cell path below: it goes to the list
+ * `dlist` with its number of work items in `ditems`.
  */
 #define PPB_T 256
+#define PPC_T 256
 __device__ static inline int pp_fix(int IX, int cc, int32_t code) {
   return (cc - 1) * (4 * 32768) + (int)(2 * cube_ucode(IX, code) + 1) * (IX == 1 ? 256 : 1);
 }
 __device__ static inline uint64_t pp_pack(const int r[3]) {
   return (uint64_t)(uint32_t)r[0] | ((uint64_t)(uint32_t)r[1] << 21) | ((uint64_t)(uint32_t)r[2] << 42);
+}
+/* x21 from a staged word and the particle's own fixed-point position */
+__device__ __forceinline__ void pp_unpack_d(uint64_t w, const int X1[3], double x21[3]) {
+  const int X2[3] = {(int)(w & 0x1FFFFF), (int)((w >> 21) & 0x1FFFFF), (int)(w >> 42)};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) x21[a] = (double)(X2[a] - X1[a]) * (1.0 / 32768.0);
+}
+/* the kick by the summed force and the re-encoded velocity (:116-123) */
+template <int IV>
+__device__ __forceinline__ void pp_kick(typename IntOf<IV>::T *__restrict__ vp, int64_t p, double vreal[3],
+                                        const double ft[3], double a_mid, double dt, double venc,
+                                        const double *__restrict__ vtab, double &f2m) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    vreal[a] = vreal[a] + (((ft[a] * a_mid) * dt) / 6.0) / (double)CUBE_PI_F;
+    vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v_tab<IV>(venc, vreal[a], vtab);
+  }
+  const double f2 = (ft[0] * ft[0] + ft[1] * ft[1]) + ft[2] * ft[2];
+  f2m = f2 > f2m ? f2 : f2m;
+}
+/* max |f|^2 and the particle count of a workgroup: wave shuffles, the
+   waves through LDS, one global atomic pair (max and sum do not depend on
+   the order) */
+template <int NT>
+__device__ __forceinline__ void pp_reduce(double f2m, unsigned long long ntest, unsigned long long *__restrict__ out) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    f2m = fmax(f2m, __shfl_xor(f2m, o, 64));
+    ntest += __shfl_xor(ntest, o, 64);
+  }
+  __shared__ double wf2[NT / 64];
+  __shared__ unsigned long long wn[NT / 64];
+  if ((tid & 63) == 0) {
+    wf2[tid >> 6] = f2m;
+    wn[tid >> 6] = ntest;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < NT / 64; ++w) {
+      f2m = fmax(f2m, wf2[w]);
+      ntest += wn[w];
+    }
+    atomicMax(cube_slot(out), (unsigned long long)__double_as_longlong(f2m));
+    atomicAdd(cube_slot(out + CUBE_SLOTS), ntest);
+  }
 }
 
 template <int IX, int IV>
@@ -190,9 +239,8 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
                                                   const double *__restrict__ vdec, const double *__restrict__ vtab, Geo g, int nvh, int pr,
                                                   double rsoft, double mass, double a_mid, double dt, double venc,
                                                   int maxs, int fdiv, unsigned long long *__restrict__ out,
-                                                  int32_t *__restrict__ dlist, unsigned long long *__restrict__ dcount,
-                                                  const int32_t *__restrict__ list,
-                                                  const unsigned long long *__restrict__ nlist) {
+                                                  int32_t *__restrict__ dlist, int32_t *__restrict__ ditems,
+                                                  unsigned long long *__restrict__ dcount) {
   extern __shared__ uint64_t smem64[];
   const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
   uint64_t *pos = smem64;                                            /* packed positions (maxs) */
@@ -200,14 +248,12 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   __shared__ int part[PPB_T / 64];
   __shared__ int64_t cst[16][5];
   __shared__ int rowoff[17];
+  __shared__ int wide; /* a staged coarse cell too full for the 16-bit counts */
   const int tid = threadIdx.x, nt = g.nt, nb = nt / 4, nnt = g.nnt;
   const int sh = IX == 1 ? 6 : 14;
   double f2m = 0;
   unsigned long long ntest = 0;
-  /* first launch: block blockIdx.x; second launch: the deferred blocks */
-  const int64_t nw = list ? (int64_t)*nlist : (int64_t)gridDim.x;
-  for (int64_t w = blockIdx.x; w < nw; w += gridDim.x) {
-  const int64_t b = list ? (int64_t)list[w] : w;
+  const int64_t b = blockIdx.x;
   const int bx = (int)(b % nb), by = (int)((b / nb) % nb), bz = (int)((b / ((int64_t)nb * nb)) % nb);
   const int t = (int)(b / ((int64_t)nb * nb * nb));
   const int tt[3] = {t % nnt, (t / nnt) % nnt, t / (nnt * nnt)};
@@ -217,6 +263,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   const int X0[3] = {(flo[0] - 1) * 32768, (flo[1] - 1) * 32768, (flo[2] - 1) * 32768};
 
   for (int i = tid - 1; i < nreg; i += PPB_T) cur[i] = 0;
+  if (tid == 0) wide = 0;
   if (tid < 16) {
     const int ry = tid % 4, rz = tid / 4;
     const int64_t c0 = g.cid(B0[0], B0[1] + ry, B0[2] + rz, t);
@@ -236,6 +283,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     p0 = cstart[cq];
   }
   __syncthreads();
+  if (np > 65535) wide = 1;
   /* the region cell of a particle of this thread's staged cell, or -1 */
   auto region = [&](const int32_t code[3]) -> int {
     bool in = true;
@@ -276,37 +324,43 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   for (int i = i0; i < i1; ++i) s += cur[i];
   int nstage;
   const int incl = block_incl_scan<PPB_T / 64>(s, part, nstage);
-  const bool staged = nstage <= maxs; /* uniform over the block */
   s = incl - s;
-  if (!staged && dlist) { /* uniform: the block goes to the second launch */
-    if (tid == 0) dlist[atomicAdd(dcount, 1ull)] = (int32_t)b;
-    __syncthreads();
-    continue;
+  if (nstage > maxs || wide) { /* uniform: the block goes to the cell path */
+    if (tid < 64) {
+      /* work items: one per PPC_T particles of each of its 64 coarse cells */
+      int items = (int)((cst[tid >> 2][(tid & 3) + 1] - cst[tid >> 2][tid & 3] + PPC_T - 1) / PPC_T);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) items += __shfl_xor(items, o, 64);
+      if (tid == 0) {
+        const unsigned long long d = atomicAdd(dcount, 1ull);
+        dlist[d] = (int32_t)b;
+        ditems[d] = items;
+      }
+    }
+    return;
   }
   for (int i = i0; i < i1; ++i) {
     const int cnt = cur[i];
-    cur[i] = (uint16_t)(staged ? s : 0);
+    cur[i] = (uint16_t)s;
     s += cnt;
   }
   __syncthreads();
-  if (staged) {
-    /* chain order inside a fine cell is decreasing l: walk l downwards and
-       append; afterwards cur[f] is the end of fine cell f's list */
-    for (int l0 = np - 1; l0 >= 0; l0 -= 4) {
-      int32_t c4[4][3];
-      codes4(l0, -1, c4);
+  /* chain order inside a fine cell is decreasing l: walk l downwards and
+     append; afterwards cur[f] is the end of fine cell f's list */
+  for (int l0 = np - 1; l0 >= 0; l0 -= 4) {
+    int32_t c4[4][3];
+    codes4(l0, -1, c4);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (l0 - u < 0) break;
-        const int ri = region(c4[u]);
-        if (ri < 0) continue;
-        const int slot = cur[ri];
-        cur[ri] = (uint16_t)(slot + 1);
-        int r[3];
+    for (int u = 0; u < 4; ++u) {
+      if (l0 - u < 0) break;
+      const int ri = region(c4[u]);
+      if (ri < 0) continue;
+      const int slot = cur[ri];
+      cur[ri] = (uint16_t)(slot + 1);
+      int r[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) r[a] = pp_fix(IX, sc[a], c4[u][a]) - X0[a];
-        pos[slot] = pp_pack(r);
-      }
+      for (int a = 0; a < 3; ++a) r[a] = pp_fix(IX, sc[a], c4[u][a]) - X0[a];
+      pos[slot] = pp_pack(r);
     }
   }
   if (tid == 0) {
@@ -331,144 +385,299 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     const int loc[3] = {B0[0] + 1 + cx, B0[1] + 1 + row % 4, B0[2] + 1 + row / 4};
     double vreal[3], ft[3] = {0, 0, 0};
     int X1[3], gf[3];
-    int32_t pcode[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int32_t code = (int32_t)xp[3 * p + a];
-      pcode[a] = code;
       X1[a] = pp_fix(IX, loc[a], code) - X0[a];
       gf[a] = 4 * (loc[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
       vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
     }
-    if (staged) {
-      /* the 2pr+1 fine cells ii-pr..ii+pr of one (kk,jj) are adjacent in
-         the region order, so their lists form one contiguous range of pos:
-         (2pr+1)^2 ranges, walked kk-major then jj, hold every neighbour in
-         the reference's (kk,jj,ii, chain) order.  One flat loop over the
-         particle's pairs keeps a wave's lanes together: it runs the largest
-         pair count of the wave, not 27 times the largest per-cell count. */
-      const int RW = 2 * pr + 1;
-      const int f00 = ((gf[2] - pr - flo[2]) * RF + (gf[1] - pr - flo[1])) * RF + (gf[0] - pr - flo[0]);
-      /* row (kk, jj) starts at f00 + (kk*RF + jj)*RF: stepped by counters,
-         not by dividing by the run-time RW */
-      int n = 0;
-      for (int kk = 0; kk < RW; ++kk)
-        for (int jj = 0; jj < RW; ++jj) {
-          const int f0 = f00 + (kk * RF + jj) * RF;
-          n += (int)cur[f0 + RW - 1] - (int)cur[f0 - 1];
-        }
-      int f0 = f00 - RF, jr = -1, base = 0, cend = 0; /* pair t < cend lives at pos slot base + t */
-      for (int t = 0; t < n; ++t) {
-        while (t >= cend) {
-          if (++jr == RW) { /* next kk */
-            jr = 0;
-            f0 += (RF - RW + 1) * RF;
-          } else {
-            f0 += RF;
-          }
-          const int s0 = cur[f0 - 1], e = cur[f0 + RW - 1];
-          base = s0 - cend;
-          cend += e - s0;
-        }
-        const uint64_t w = pos[base + t];
-        const int X2[3] = {(int)(w & 0x1FFFFF), (int)((w >> 21) & 0x1FFFFF), (int)(w >> 42)};
-        double x21[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) x21[a] = (double)(X2[a] - X1[a]) * (1.0 / 32768.0);
-        double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
-        if (rmag == 0.0) rmag = 1.0;
-        const double rcut = rmag / 16.0;
-        const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
-        const double r3 = powi3(rmag);
-        double qt[3];
-        quot3(x21, r3, qt, fdiv);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          double fp = (mass * qt[a]) * pcut;
-          if (!(rmag > rsoft)) fp = fp * 0.0;
-          ft[a] = ft[a] + fp;
-        }
+    /* the 2pr+1 fine cells ii-pr..ii+pr of one (kk,jj) are adjacent in the
+       region order, so their lists form one contiguous range of pos:
+       (2pr+1)^2 ranges, walked kk-major then jj, hold every neighbour in
+       the reference's (kk,jj,ii, chain) order.  One flat loop over the
+       particle's pairs keeps a wave's lanes together: it runs the largest
+       pair count of the wave, not 27 times the largest per-cell count. */
+    const int RW = 2 * pr + 1;
+    const int f00 = ((gf[2] - pr - flo[2]) * RF + (gf[1] - pr - flo[1])) * RF + (gf[0] - pr - flo[0]);
+    /* row (kk, jj) starts at f00 + (kk*RF + jj)*RF: stepped by counters,
+       not by dividing by the run-time RW */
+    int n = 0;
+    for (int kk = 0; kk < RW; ++kk)
+      for (int jj = 0; jj < RW; ++jj) {
+        const int f0 = f00 + (kk * RF + jj) * RF;
+        n += (int)cur[f0 + RW - 1] - (int)cur[f0 - 1];
       }
-    } else {
-    double x1[3]; /* FP64 tile-frame position for the walk over the global lists */
-#pragma unroll
-    for (int a = 0; a < 3; ++a) x1[a] = (double)(4 * (loc[a] - 1)) + 4.0 * cube_xoff(IX, pcode[a]);
-    for (int kk = gf[2] - pr; kk <= gf[2] + pr; ++kk)
-      for (int jj = gf[1] - pr; jj <= gf[1] + pr; ++jj)
-        for (int ii = gf[0] - pr; ii <= gf[0] + pr; ++ii) {
-          {
-            const int fc[3] = {ii, jj, kk};
-            int cc[3], sub[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              cc[a] = (int)floorf((float)(fc[a] - 1) / 4.0f) + 1;
-              sub[a] = (fc[a] - 1) - 4 * (cc[a] - 1);
-            }
-            const int64_t cq =
-                cid_global(g, tt[0] * nt + cc[0] - 1, tt[1] * nt + cc[1] - 1, tt[2] * nt + cc[2] - 1);
-            const int np = rhoc[cq];
-            const int64_t q0 = cstart[cq];
-            for (int l = np - 1; l >= 0; --l) {
-              const int64_t q = q0 + l;
-              const int32_t cd[3] = {(int32_t)xp[3 * q], (int32_t)xp[3 * q + 1], (int32_t)xp[3 * q + 2]};
-              if ((int)(cube_ucode(IX, cd[0]) >> sh) != sub[0] || (int)(cube_ucode(IX, cd[1]) >> sh) != sub[1] ||
-                  (int)(cube_ucode(IX, cd[2]) >> sh) != sub[2])
-                continue;
-              double x21[3];
-#pragma unroll
-              for (int a = 0; a < 3; ++a)
-                x21[a] = ((double)(4 * (cc[a] - 1)) + 4.0 * cube_xoff(IX, cd[a])) - x1[a];
-              double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
-              if (rmag == 0.0) rmag = 1.0;
-              const double rcut = rmag / 16.0;
-              const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
-              const double r3 = powi3(rmag);
-              double qt[3];
-              quot3(x21, r3, qt, fdiv);
-#pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                double fp = (mass * qt[a]) * pcut;
-                if (!(rmag > rsoft)) fp = fp * 0.0;
-                ft[a] = ft[a] + fp;
-              }
-            }
-          }
+    int f0 = f00 - RF, jr = -1, base = 0, cend = 0; /* pair t < cend lives at pos slot base + t */
+    for (int t2 = 0; t2 < n; ++t2) {
+      while (t2 >= cend) {
+        if (++jr == RW) { /* next kk */
+          jr = 0;
+          f0 += (RF - RW + 1) * RF;
+        } else {
+          f0 += RF;
         }
+        const int s0 = cur[f0 - 1], e = cur[f0 + RW - 1];
+        base = s0 - cend;
+        cend += e - s0;
+      }
+      double x21[3];
+      pp_unpack_d(pos[base + t2], X1, x21);
+      pp_add(x21, rsoft, mass, fdiv, ft);
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      vreal[a] = vreal[a] + (((ft[a] * a_mid) * dt) / 6.0) / (double)CUBE_PI_F;
-      vp[3 * p + a] = (typename IntOf<IV>::T)cube_encode_v_tab<IV>(venc, vreal[a], vtab);
-    }
-    const double f2 = (ft[0] * ft[0] + ft[1] * ft[1]) + ft[2] * ft[2];
-    f2m = f2 > f2m ? f2 : f2m;
+    pp_kick<IV>(vp, p, vreal, ft, a_mid, dt, venc, vtab, f2m);
     ++ntest;
   }
-  __syncthreads(); /* the block's tables are read until here */
+  pp_reduce<PPB_T>(f2m, ntest, out);
+}
+
+/* dlist's item counts to exclusive prefix sums in place, the total after
+   the last (one workgroup) */
+__global__ __launch_bounds__(1024) void k_pp_item_scan(int32_t *__restrict__ d,
+                                                       const unsigned long long *__restrict__ dcount) {
+  __shared__ int part[16];
+  const int n = (int)*dcount, tid = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int i0 = min(tid * chunk, n), i1 = min(i0 + chunk, n);
+  int s = 0;
+  for (int i = i0; i < i1; ++i) s += d[i];
+  int tot;
+  const int incl = block_incl_scan<16>(s, part, tot);
+  s = incl - s;
+  for (int i = i0; i < i1; ++i) {
+    const int v = d[i];
+    d[i] = s;
+    s += v;
   }
-  /* max |f|^2 and the particle count: wave shuffles, the waves through
-     LDS, one global atomic pair per workgroup (max and sum do not depend
-     on the order) */
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    f2m = fmax(f2m, __shfl_xor(f2m, o, 64));
-    ntest += __shfl_xor(ntest, o, 64);
-  }
-  __shared__ double wf2[PPB_T / 64];
-  __shared__ unsigned long long wn[PPB_T / 64];
-  if ((tid & 63) == 0) {
-    wf2[tid >> 6] = f2m;
-    wn[tid >> 6] = ntest;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    for (int w = 1; w < PPB_T / 64; ++w) {
-      f2m = fmax(f2m, wf2[w]);
-      ntest += wn[w];
+  if (tid == 0) d[n] = tot;
+}
+
+/*
+ * The cell path, for the blocks of clustered regions: a work item is one
+ * coarse cell C of a deferred block and up to PPC_T of its particles
+ * (chunk c: particles c*PPC_T ...), so a dense block spreads over as many
+ * workgroups as it has particles / PPC_T, and workgroups claim items from
+ * a counter as they finish.  An item stages the (4+2*pr)^3 fine cells
+ * around C, which lie in the 3^3 coarse cells C-1..C+1, in the staged
+ * order above (region order of fine cells, chain order inside each).  A
+ * particle's neighbours are a subsequence of that order in the
+ * reference's (kk,jj,ii, chain) order, so when the region holds more than
+ * `cap` particles it is staged `cap` slots at a time and each particle
+ * carries its sums from one window to the next: the same pairs in the
+ * same order, the same bits, in bounded LDS.
+ *
+ * Placement keeps chain order with wave-wide ranks: a wave takes 64
+ * particles of a staged coarse cell in decreasing l, lanes with the same
+ * fine cell find each other by ballots over the bits of its index, and a
+ * lane's slot is the cell's running count plus the number of such lanes
+ * before it.  Each staged coarse cell belongs to one wave, and each fine
+ * cell to one coarse cell, so no two waves touch one count.
+ */
+template <int IX, int IV>
+__global__ __launch_bounds__(PPC_T) void k_pp_cell(const typename IntOf<IX>::T *__restrict__ xp,
+                                                   typename IntOf<IV>::T *__restrict__ vp,
+                                                   const int32_t *__restrict__ rhoc,
+                                                   const int64_t *__restrict__ cstart,
+                                                   const double *__restrict__ vdec, const double *__restrict__ vtab, Geo g,
+                                                   int nvh, int pr, double rsoft, double mass, double a_mid, double dt,
+                                                   double venc, int cap, int fdiv, unsigned long long *__restrict__ out,
+                                                   const int32_t *__restrict__ dlist, const int32_t *__restrict__ dpre,
+                                                   const unsigned long long *__restrict__ dcount,
+                                                   unsigned long long *__restrict__ claim) {
+  extern __shared__ uint64_t smem64[];
+  const int RF = 4 + 2 * pr, nreg = RF * RF * RF; /* nreg < 2048 for pr <= 4 */
+  uint64_t *pos = smem64;                           /* one window of staged positions (cap) */
+  int *off = reinterpret_cast<int *>(smem64 + cap); /* nreg + 1: first slot of each fine cell */
+  int *cnt = off + nreg + 1;                        /* nreg: per fine cell, counted or placed so far */
+  __shared__ int part[PPC_T / 64];
+  __shared__ int s_item, s_blk, s_cell, s_chunk;
+  __shared__ int snp[27];
+  __shared__ int64_t sp0[27];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nt = g.nt, nb = nt / 4, nnt = g.nnt;
+  const int sh = IX == 1 ? 6 : 14;
+  const int ndef = (int)*dcount;
+  const int total = dpre[ndef];
+  double f2m = 0;
+  unsigned long long ntest = 0;
+  for (;;) {
+    if (tid == 0) {
+      s_item = (int)atomicAdd(claim, 1ull);
+      s_cell = -1;
     }
-    atomicMax(cube_slot(out), (unsigned long long)__double_as_longlong(f2m));
-    atomicAdd(cube_slot(out + CUBE_SLOTS), ntest);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= total) break; /* uniform */
+    if (wv == 0) {
+      /* the deferred block: dpre[w] <= item < dpre[w+1], 64 probes a round */
+      int lo = 0, hi = ndef;
+      while (hi - lo > 1) {
+        const int qp = lo + 1 + (int)(((int64_t)(hi - lo - 1) * lane) / 64);
+        const uint64_t m = __ballot(dpre[qp] <= item);
+        const int k = __popcll(m);
+        const int qlo = __shfl(qp, max(k - 1, 0), 64), qhi = __shfl(qp, min(k, 63), 64);
+        if (k > 0) lo = qlo;
+        hi = k < 64 ? qhi : hi;
+      }
+      /* the cell and chunk: scan of the 64 cells' chunk counts */
+      const int b = dlist[lo];
+      const int bx = b % nb, by = (b / nb) % nb, bz = (b / (nb * nb)) % nb, tb = b / (nb * nb * nb);
+      const int64_t cl = g.cid(4 * bx + (lane & 3), 4 * by + ((lane >> 2) & 3), 4 * bz + (lane >> 4), tb);
+      const int nch = (rhoc[cl] + PPC_T - 1) / PPC_T;
+      const int incl = wave_incl_scan(nch), k = item - dpre[lo];
+      if (k >= incl - nch && k < incl) {
+        s_blk = b;
+        s_cell = lane;
+        s_chunk = k - (incl - nch);
+      }
+    }
+    __syncthreads();
+    const int b = s_blk, j = s_cell, c = s_chunk;
+    if (j < 0) continue; /* uniform; cannot happen while cstart is the prefix sum of rhoc */
+    const int bx = b % nb, by = (b / nb) % nb, bz = (b / (nb * nb)) % nb, tb = b / (nb * nb * nb);
+    const int tt[3] = {tb % nnt, (tb / nnt) % nnt, tb / (nnt * nnt)};
+    const int C[3] = {4 * bx + 1 + (j & 3), 4 * by + 1 + ((j >> 2) & 3), 4 * bz + 1 + (j >> 4)}; /* tile frame */
+    const int flo[3] = {4 * (C[0] - 1) + 1 - pr, 4 * (C[1] - 1) + 1 - pr, 4 * (C[2] - 1) + 1 - pr};
+    const int X0[3] = {(flo[0] - 1) * 32768, (flo[1] - 1) * 32768, (flo[2] - 1) * 32768};
+    if (tid < 27) {
+      const int64_t cq = cid_global(g, tt[0] * nt + C[0] - 2 + tid % 3, tt[1] * nt + C[1] - 2 + (tid / 3) % 3,
+                                    tt[2] * nt + C[2] - 2 + tid / 9);
+      snp[tid] = rhoc[cq];
+      sp0[tid] = cstart[cq];
+    }
+    for (int i = tid; i < nreg; i += PPC_T) cnt[i] = 0;
+    __syncthreads();
+    /* staged coarse cell q = C - 1 + (q%3, q/3%3, q/9); its particle's
+       region cell, or -1 */
+    auto region = [&](int q, const int32_t code[3]) -> int {
+      const int sq[3] = {C[0] - 1 + q % 3, C[1] - 1 + (q / 3) % 3, C[2] - 1 + q / 9};
+      bool in = true;
+      int r[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        r[a] = 4 * (sq[a] - 1) + (int)(cube_ucode(IX, code[a]) >> sh) + 1 - flo[a];
+        in = in && r[a] >= 0 && r[a] < RF;
+      }
+      return in ? (r[2] * RF + r[1]) * RF + r[0] : -1;
+    };
+    /* count per fine cell (order-free: LDS atomics) */
+    for (int q = 0; q < 27; ++q) {
+      const int np = snp[q];
+      const int64_t p0 = sp0[q];
+      for (int l = tid; l < np; l += PPC_T) {
+        const int32_t cd[3] = {(int32_t)xp[3 * (p0 + l)], (int32_t)xp[3 * (p0 + l) + 1], (int32_t)xp[3 * (p0 + l) + 2]};
+        const int ri = region(q, cd);
+        if (ri >= 0) atomicAdd(&cnt[ri], 1);
+      }
+    }
+    __syncthreads();
+    /* exclusive scan into off; off[nreg] = particles staged */
+    const int chunk = (nreg + PPC_T - 1) / PPC_T;
+    const int i0 = min(tid * chunk, nreg), i1 = min(i0 + chunk, nreg);
+    int s = 0;
+    for (int i = i0; i < i1; ++i) s += cnt[i];
+    int nstage;
+    const int incl = block_incl_scan<PPC_T / 64>(s, part, nstage);
+    s = incl - s;
+    for (int i = i0; i < i1; ++i) {
+      off[i] = s;
+      s += cnt[i];
+    }
+    if (tid == 0) off[nreg] = nstage;
+    /* this thread's particle: number c*PPC_T + tid of C */
+    const int lp = c * PPC_T + tid;
+    const bool act = lp < snp[13];
+    const int64_t p = sp0[13] + lp;
+    int X1[3] = {0, 0, 0}, f00 = 0;
+    double vreal[3] = {0, 0, 0}, ft[3] = {0, 0, 0};
+    if (act) {
+      int gf[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int32_t code = (int32_t)xp[3 * p + a];
+        X1[a] = pp_fix(IX, C[a], code) - X0[a];
+        gf[a] = 4 * (C[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
+        vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
+      }
+      f00 = ((gf[2] - pr - flo[2]) * RF + (gf[1] - pr - flo[1])) * RF + (gf[0] - pr - flo[0]);
+    }
+    const int RW = 2 * pr + 1;
+    for (int w0 = 0; w0 < nstage; w0 += cap) {
+      const int w1 = w0 + cap;
+      __syncthreads(); /* off is complete; the last window's pairs are done */
+      for (int i = tid; i < nreg; i += PPC_T) cnt[i] = 0;
+      __syncthreads();
+      for (int q = wv; q < 27; q += PPC_T / 64) {
+        const int np = snp[q];
+        const int64_t p0 = sp0[q];
+        const int sq[3] = {C[0] - 1 + q % 3, C[1] - 1 + (q / 3) % 3, C[2] - 1 + q / 9};
+        for (int base = 0; base < np; base += 64) { /* uniform over the wave */
+          const int l = np - 1 - (base + lane);
+          int ri = -1;
+          int32_t cd[3] = {0, 0, 0};
+          if (l >= 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) cd[a] = (int32_t)xp[3 * (p0 + l) + a];
+            ri = region(q, cd);
+          }
+          const unsigned key = (unsigned)(ri + 1);
+          uint64_t m = ~0ull;
+#pragma unroll
+          for (int bit = 0; bit < 11; ++bit) {
+            const uint64_t bb = __ballot((key >> bit) & 1u);
+            m &= ((key >> bit) & 1u) ? bb : ~bb;
+          }
+          int done = 0;
+          if (ri >= 0) {
+            done = cnt[ri];
+            const int slot = off[ri] + done + __popcll(m & ((1ull << lane) - 1ull));
+            if (slot >= w0 && slot < w1) {
+              int r[3];
+#pragma unroll
+              for (int a = 0; a < 3; ++a) r[a] = pp_fix(IX, sq[a], cd[a]) - X0[a];
+              pos[slot - w0] = pp_pack(r);
+            }
+          }
+          /* every lane has read its count: the group's last lane advances it */
+          if (ri >= 0 && ((m >> lane) >> 1) == 0ull) cnt[ri] = done + __popcll(m);
+        }
+      }
+      __syncthreads();
+      if (act) {
+        /* the particle's (2pr+1)^2 row ranges clipped to [w0, w1), one flat
+           loop as in the block kernel */
+        int n = 0;
+        for (int kk = 0; kk < RW; ++kk)
+          for (int jj = 0; jj < RW; ++jj) {
+            const int f0 = f00 + (kk * RF + jj) * RF;
+            n += max(0, min(off[f0 + RW], w1) - max(off[f0], w0));
+          }
+        int f0 = f00 - RF, jr = -1, base = 0, cend = 0;
+        for (int t2 = 0; t2 < n; ++t2) {
+          while (t2 >= cend) {
+            if (++jr == RW) {
+              jr = 0;
+              f0 += (RF - RW + 1) * RF;
+            } else {
+              f0 += RF;
+            }
+            const int s0 = max(off[f0], w0), e = min(off[f0 + RW], w1);
+            base = s0 - w0 - cend;
+            cend += max(e - s0, 0);
+          }
+          double x21[3];
+          pp_unpack_d(pos[base + t2], X1, x21);
+          pp_add(x21, rsoft, mass, fdiv, ft);
+        }
+      }
+    }
+    if (act) {
+      pp_kick<IV>(vp, p, vreal, ft, a_mid, dt, venc, vtab, f2m);
+      ++ntest;
+    }
+    __syncthreads(); /* the item's tables are read until here */
   }
+  pp_reduce<PPC_T>(f2m, ntest, out);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -485,44 +694,39 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
   if (c->nt % 4 == 0 && pr >= 0 && pr <= 4) {
     const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
     const size_t table = (((size_t)nreg + 1) * 2 + 7) / 8 * 8; /* uint16 per fine cell, plus cur[-1] */
-    /* first launch: staged positions (8 bytes each) and the fine-cell
-       table in about 28 KB, so five workgroups share a CU's LDS */
+    /* block kernel: staged positions (8 bytes each) and the fine-cell table
+       in about 28 KB, so five workgroups share a CU's LDS */
     int maxs = pr <= 1 ? 2048 : (pr == 2 ? 1024 : 512);
-    /* second launch: most of the LDS a workgroup may take, for the blocks
-       the first one deferred (uint16 offsets cap it at 65535) */
-    int lds = 65536;
-    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess) lds = 65536;
-    int maxb = (int)std::min<int64_t>(16384, ((int64_t)lds - 4096 - (int64_t)table) / 8);
-    /* lower caps send more workgroups down the second launch or the
-       unstaged walk (used by the tests to cover every path in one run) */
-    if (const char *e = getenv("CUBE_PP_STAGE_MAX")) maxs = std::min(maxs, atoi(e));
-    if (const char *e = getenv("CUBE_PP_BIG_MAX")) maxb = std::min(maxb, atoi(e));
-    const bool two = maxb > maxs;
+    /* cell path: a window of 4096 staged positions and two int tables
+       (under 47 KB at pr = 4) */
+    const int RC = 4 + 2 * pr, nregc = RC * RC * RC;
+    int cap = 4096;
+    /* lower values send more blocks down the cell path and stage its
+       regions in more windows (used by the tests to cover every path) */
+    if (const char *e = getenv("CUBE_PP_STAGE_MAX")) maxs = std::max(0, std::min(maxs, atoi(e)));
+    if (const char *e = getenv("CUBE_PP_CELL_CAP")) cap = std::max(1, std::min(cap, atoi(e)));
     const int64_t nb = c->nt / 4, nblk = nb * nb * nb * c->ntiles;
-    unsigned long long *dcount = (unsigned long long *)c->ured + 24;
-    if (two) {
-      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, dcount, 1);
-      CUBE_LAUNCH_CHECK();
-    }
+    unsigned long long *dcount = (unsigned long long *)c->ured + 24, *claim = (unsigned long long *)c->ured + 25;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, dcount, 2);
+    CUBE_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3((unsigned)nblk), dim3(PPB_T), (size_t)maxs * 8 + table, c->stream,
                        (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
                        c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p, (double)a_mid,
-                       (double)dt, cube_vconst(sigma_vi), maxs, fdiv, out, two ? c->ppdefer : nullptr, dcount, nullptr,
-                       nullptr);
+                       (double)dt, cube_vconst(sigma_vi), maxs, fdiv, out, c->ppdefer, c->ppitems, dcount);
     CUBE_LAUNCH_CHECK();
-    if (two) {
-      /* a fixed grid of one workgroup per CU-slot walks however many blocks
-         were deferred (read on the device: no host round trip) */
-      int ncu = 256;
-      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) ncu = 256;
-      const unsigned grid = (unsigned)std::min<int64_t>(nblk, 2 * (int64_t)ncu);
-      hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3(grid), dim3(PPB_T), (size_t)maxb * 8 + table, c->stream,
-                         (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
-                         c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p,
-                         (double)a_mid, (double)dt, cube_vconst(sigma_vi), maxb, fdiv, out, nullptr, nullptr,
-                         c->ppdefer, dcount);
-      CUBE_LAUNCH_CHECK();
-    }
+    hipLaunchKernelGGL(k_pp_item_scan, dim3(1), dim3(1024), 0, c->stream, c->ppitems, dcount);
+    CUBE_LAUNCH_CHECK();
+    /* a fixed grid claims the items (the count stays on the device: no
+       host round trip) */
+    int ncu = 256;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) ncu = 256;
+    const size_t lds = (size_t)cap * 8 + (size_t)(2 * nregc + 1) * 4;
+    hipLaunchKernelGGL((k_pp_cell<IX, IV>), dim3((unsigned)(4 * ncu)), dim3(PPC_T), lds, c->stream,
+                       (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
+                       c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p,
+                       (double)a_mid, (double)dt, cube_vconst(sigma_vi), cap, fdiv, out, c->ppdefer, c->ppitems, dcount,
+                       claim);
+    CUBE_LAUNCH_CHECK();
     return CUBE_OK;
   }
   const int64_t nrows = c->ncells / c->nt;

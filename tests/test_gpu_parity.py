@@ -153,21 +153,55 @@ def test_kicks_bit_exact_given_forces(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=_ids)
-@pytest.mark.parametrize("stage_max,big_max", [(None, None), ("0", None), ("40", "0"), ("0", "0"), ("40", "200")])
-def test_pp_bit_exact(case, stage_max, big_max, monkeypatch):
+@pytest.mark.parametrize("stage_max,cell_cap", [(None, None), ("0", None), ("40", "16"), ("0", "1"), ("40", "200")])
+def test_pp_bit_exact(case, stage_max, cell_cap, monkeypatch):
     """PP stages each 4^3-cell block's neighbourhood in LDS; blocks with more
-    than CUBE_PP_STAGE_MAX particles there go to a second launch with a
-    larger staging area (CUBE_PP_BIG_MAX), and blocks beyond that walk the
-    global lists.  Every mix of the three paths must match the oracle bit
-    for bit."""
+    than CUBE_PP_STAGE_MAX particles there go to the cell path (work items of
+    one coarse cell and up to 256 of its particles), which stages its
+    neighbourhood CUBE_PP_CELL_CAP particles at a time.  Every mix of the
+    paths and window sizes must match the oracle bit for bit."""
     if stage_max is not None:
         monkeypatch.setenv("CUBE_PP_STAGE_MAX", stage_max)
-    if big_max is not None:
-        monkeypatch.setenv("CUBE_PP_BIG_MAX", big_max)
+    if cell_cap is not None:
+        monkeypatch.setenv("CUBE_PP_CELL_CAP", cell_cap)
     st, oc, g = _pair(case, pp_on=True)
     m = st["mass_p"]
     f_o = oc.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
     f_g = g.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
+    _same_state(g.get_state(), oc.get_state())
+    assert f_g == f_o
+
+
+@pytest.mark.parametrize("stage_max,cell_cap", [(None, None), ("0", None), (None, "300")])
+def test_pp_clustered_bit_exact(stage_max, cell_cap, monkeypatch):
+    """A collapsed clump (6000 particles, Gaussian with sigma 0.7 coarse
+    cells) in a uniform background of 10000: the blocks around it go to the
+    cell path, its coarse cells hold several 256-particle items each, and
+    their neighbourhoods hold more than one staging window.  Velocities and
+    max |f|^2 match the oracle bit for bit."""
+    from cafproject_amd.cube import Cube
+    if stage_max is not None:
+        monkeypatch.setenv("CUBE_PP_STAGE_MAX", stage_max)
+    if cell_cap is not None:
+        monkeypatch.setenv("CUBE_PP_CELL_CAP", cell_cap)
+    nc, nnt = 16, 1
+    rng = np.random.default_rng(5)
+    clump = np.array([7.3, 8.1, 8.6]) + rng.normal(0, 0.7, (6000, 3))
+    rest = rng.uniform(0, nc, (10000, 3))
+    pos = np.concatenate([clump, rest]) % nc
+    vel = rng.normal(0, 0.3, (16000, 3)).astype(np.float32)
+    sig = 0.3
+    g = Cube(nc, nnt, pp_on=True)
+    g.set_state_xv(pos, vel, sig)
+    oc = O.OracleCube(nc, nnt, pp_on=True)
+    oc.state_from_xv(pos, vel.astype(np.float64), None, sig)
+    oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+    so = oc.get_state()
+    _same_state(g.get_state(), so)
+    assert so["rhoc"].max() > 512  # coarse cells of three or more items
+    m = float(np.float32((4 * nc) ** 3 / 16000))
+    f_o = oc.ext_pp_force(0.05, 2.0, sig, m)
+    f_g = g.ext_pp_force(0.05, 2.0, sig, m)
     _same_state(g.get_state(), oc.get_state())
     assert f_g == f_o
 
@@ -187,16 +221,16 @@ def test_pp_three_divisions_bit_exact(case, monkeypatch):
 
 
 @pytest.mark.parametrize("pp_range", [0, 2])
-@pytest.mark.parametrize("stage_max", [None, "0", "00"])
-def test_pp_other_ranges_bit_exact(pp_range, stage_max, monkeypatch):
+@pytest.mark.parametrize("stage_max,cell_cap", [(None, None), ("0", None), ("0", "5")])
+def test_pp_other_ranges_bit_exact(pp_range, stage_max, cell_cap, monkeypatch):
     """pp_range 0 and 2: 1 and 25 (kk, jj) rows of 1 and 5 fine cells per
-    particle; staged, deferred to the large-LDS launch ("0"), and global
-    walks ("00": both caps zero)."""
+    particle; staged per block, on the cell path ("0"), and on the cell path
+    in windows of 5 particles."""
     from cafproject_amd.cube import Cube
     if stage_max is not None:
         monkeypatch.setenv("CUBE_PP_STAGE_MAX", stage_max)
-        if stage_max == "00":
-            monkeypatch.setenv("CUBE_PP_BIG_MAX", "0")
+    if cell_cap is not None:
+        monkeypatch.setenv("CUBE_PP_CELL_CAP", cell_cap)
     nc, nnt = 24, 2
     st = zeldovich_state(nc, nnt, seed=43)
     oc = O.OracleCube(nc, nnt, pp_on=True, pp_range=pp_range)
