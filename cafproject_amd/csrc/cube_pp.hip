@@ -22,10 +22,37 @@
 
 #define PT 256
 
-__device__ static inline double powi3(double x) { return x * (x * x); }
-__device__ static inline double powi5(double x) {
-  const double x2 = x * x;
-  return x2 * (x * x2);
+/* sqrt(x) for a normal x > 0 well inside the exponent range (here
+   r^2 >= 2^-36): the reciprocal-square-root seed refined by Goldschmidt's
+   coupled iteration (g -> sqrt(x), h -> 1/(2 sqrt(x))) and two Newton
+   corrections of the residual x - g*g (Markstein), correctly rounded.  It
+   is the sequence the compiler emits for sqrt() less the rescaling of
+   extreme exponents and the special-case selects, which these operands
+   never take, so the bits are sqrt()'s. */
+__device__ __forceinline__ double pp_sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+/* 1/d for a normal d well inside the exponent range (here 2^-45 < d <
+   2^21): the reciprocal seed, two Newton steps e = 1 - d*y, y += y*e, and
+   a final residual correction, correctly rounded; the compiler's division
+   sequence for 1.0/d less its operand scaling and special-case fixup,
+   which these operands never take, so the bits are 1.0/d's. */
+__device__ __forceinline__ double pp_rcp(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  const double r = fma(-d, y, 1.0);
+  return fma(r, y, y);
 }
 /* x[a] / r3 for the three components, IEEE quotients either way.  With
    one_div a single division forms y = RN(1/r3) and each quotient is
@@ -36,7 +63,7 @@ __device__ static inline double powi5(double x) {
    division. */
 __device__ static inline void quot3(const double x[3], double r3, double q[3], bool one_div) {
   if (one_div) {
-    const double y = 1.0 / r3;
+    const double y = pp_rcp(r3);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const double q0 = x[a] * y;
@@ -51,19 +78,23 @@ __device__ static inline void quot3(const double x[3], double r3, double q[3], b
 /* the pair force of a neighbour at x21 (W/ext_pp_force.f90:100-115), added
    to ft; every PP path forms it with this code */
 __device__ __forceinline__ void pp_add(const double x21[3], double rsoft, double mass, int fdiv, double ft[3]) {
-  double rmag = sqrt((x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2]);
-  if (rmag == 0.0) rmag = 1.0;
-  const double rcut = rmag / 16.0;
-  const double pcut = (1.0 - (1.75 * powi3(rcut))) + (0.75 * powi5(rcut));
-  const double r3 = powi3(rmag);
+  const double r2 = (x21[0] * x21[0] + x21[1] * x21[1]) + x21[2] * x21[2];
+  /* rmag = 0 (the particle itself) -> 1; pp_sqrt(0) is NaN and not taken */
+  const double sq = pp_sqrt(r2);
+  const double rmag = r2 == 0.0 ? 1.0 : sq;
+  /* rcut = rmag/16 scales by a power of two, which commutes with rounding:
+     rcut^3 = r3 * 2^-12 and rcut^5 = RN(rmag^2 * r3) * 2^-20 exactly (the
+     products x*(x*x) and x^2*(x*x^2)), so 1.75*rcut^3 and 0.75*rcut^5 are single
+     roundings of (1.75*2^-12)*r3 and (0.75*2^-20)*RN(rmag^2*r3) */
+  const double rr = rmag * rmag, r3 = rmag * rr;
+  const double pcut = (1.0 - (0x1.cp-12 * r3)) + (0x1.8p-21 * (rr * r3));
   double qt[3];
   quot3(x21, r3, qt, fdiv);
+  /* at rmag <= rsoft the term is (...)*0.0 = +-0, which leaves the sum as
+     it is (a sum that starts at +0 is never -0): a zero mass gives the same */
+  const double m = rmag > rsoft ? mass : 0.0;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    double fp = (mass * qt[a]) * pcut;
-    if (!(rmag > rsoft)) fp = fp * 0.0;
-    ft[a] = ft[a] + fp;
-  }
+  for (int a = 0; a < 3; ++a) ft[a] = ft[a] + (m * qt[a]) * pcut;
 }
 
 template <int IX, int IV>
@@ -184,11 +215,14 @@ __device__ static inline int pp_fix(int IX, int cc, int32_t code) {
 __device__ static inline uint64_t pp_pack(const int r[3]) {
   return (uint64_t)(uint32_t)r[0] | ((uint64_t)(uint32_t)r[1] << 21) | ((uint64_t)(uint32_t)r[2] << 42);
 }
-/* x21 from a staged word and the particle's own fixed-point position */
-__device__ __forceinline__ void pp_unpack_d(uint64_t w, const int X1[3], double x21[3]) {
+/* x21 = (X2 - X1) * 2^-15 from a staged word and the particle's own
+   position x1 = X1 * 2^-15: X2 * 2^-15 - x1 is exact in FP64 (both are
+   multiples of 2^-15 below 2^6), so one fused multiply-add per axis forms
+   it */
+__device__ __forceinline__ void pp_unpack_d(uint64_t w, const double x1[3], double x21[3]) {
   const int X2[3] = {(int)(w & 0x1FFFFF), (int)((w >> 21) & 0x1FFFFF), (int)(w >> 42)};
 #pragma unroll
-  for (int a = 0; a < 3; ++a) x21[a] = (double)(X2[a] - X1[a]) * (1.0 / 32768.0);
+  for (int a = 0; a < 3; ++a) x21[a] = fma((double)X2[a], 0x1p-15, -x1[a]);
 }
 /* the kick by the summed force and the re-encoded velocity (:116-123) */
 template <int IV>
@@ -384,11 +418,12 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
     while (cx < 3 && cst[row][cx + 1] <= p) ++cx;
     const int loc[3] = {B0[0] + 1 + cx, B0[1] + 1 + row % 4, B0[2] + 1 + row / 4};
     double vreal[3], ft[3] = {0, 0, 0};
-    int X1[3], gf[3];
+    int gf[3];
+    double x1[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int32_t code = (int32_t)xp[3 * p + a];
-      X1[a] = pp_fix(IX, loc[a], code) - X0[a];
+      x1[a] = (double)(pp_fix(IX, loc[a], code) - X0[a]) * 0x1p-15;
       gf[a] = 4 * (loc[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
       vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
     }
@@ -422,7 +457,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
         cend += e - s0;
       }
       double x21[3];
-      pp_unpack_d(pos[base + t2], X1, x21);
+      pp_unpack_d(pos[base + t2], x1, x21);
       pp_add(x21, rsoft, mass, fdiv, ft);
     }
     pp_kick<IV>(vp, p, vreal, ft, a_mid, dt, venc, vtab, f2m);
@@ -588,14 +623,15 @@ __global__ __launch_bounds__(PPC_T) void k_pp_cell(const typename IntOf<IX>::T *
     const int lp = c * PPC_T + tid;
     const bool act = lp < snp[13];
     const int64_t p = sp0[13] + lp;
-    int X1[3] = {0, 0, 0}, f00 = 0;
+    int f00 = 0;
+    double x1[3] = {0, 0, 0};
     double vreal[3] = {0, 0, 0}, ft[3] = {0, 0, 0};
     if (act) {
       int gf[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const int32_t code = (int32_t)xp[3 * p + a];
-        X1[a] = pp_fix(IX, C[a], code) - X0[a];
+        x1[a] = (double)(pp_fix(IX, C[a], code) - X0[a]) * 0x1p-15;
         gf[a] = 4 * (C[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
         vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
       }
@@ -666,7 +702,7 @@ __global__ __launch_bounds__(PPC_T) void k_pp_cell(const typename IntOf<IX>::T *
             cend += max(e - s0, 0);
           }
           double x21[3];
-          pp_unpack_d(pos[base + t2], X1, x21);
+          pp_unpack_d(pos[base + t2], x1, x21);
           pp_add(x21, rsoft, mass, fdiv, ft);
         }
       }
