@@ -526,6 +526,8 @@ __global__ __launch_bounds__(PPC_T) void k_pp_cell(const typename IntOf<IX>::T *
   int *cnt = off + nreg + 1;                        /* nreg: per fine cell, counted or placed so far */
   __shared__ int part[PPC_T / 64];
   __shared__ int s_item, s_blk, s_cell, s_chunk;
+  __shared__ int own0[64];     /* first rank of each of C's 64 fine cells in the staged order */
+  __shared__ int tgt[PPC_T];   /* this item's particles (l in C), in the staged order */
   __shared__ int snp[27];
   __shared__ int64_t sp0[27];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -619,29 +621,25 @@ __global__ __launch_bounds__(PPC_T) void k_pp_cell(const typename IntOf<IX>::T *
       s += cnt[i];
     }
     if (tid == 0) off[nreg] = nstage;
-    /* this thread's particle: number c*PPC_T + tid of C */
-    const int lp = c * PPC_T + tid;
-    const bool act = lp < snp[13];
-    const int64_t p = sp0[13] + lp;
+    /* this thread's particle: number c*PPC_T + tid of C's particles taken
+       in the staged order (fine cell by fine cell), so that a wave's lanes
+       share fine cells and so their neighbour lists and pair counts */
+    const bool act = c * PPC_T + tid < snp[13];
+    int64_t p = 0;
     int f00 = 0;
     double x1[3] = {0, 0, 0};
     double vreal[3] = {0, 0, 0}, ft[3] = {0, 0, 0};
-    if (act) {
-      int gf[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int32_t code = (int32_t)xp[3 * p + a];
-        x1[a] = (double)(pp_fix(IX, C[a], code) - X0[a]) * 0x1p-15;
-        gf[a] = 4 * (C[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
-        vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
-      }
-      f00 = ((gf[2] - pr - flo[2]) * RF + (gf[1] - pr - flo[1])) * RF + (gf[0] - pr - flo[0]);
-    }
     const int RW = 2 * pr + 1;
     for (int w0 = 0; w0 < nstage; w0 += cap) {
       const int w1 = w0 + cap;
       __syncthreads(); /* off is complete; the last window's pairs are done */
       for (int i = tid; i < nreg; i += PPC_T) cnt[i] = 0;
+      if (w0 == 0 && wv == 0) {
+        /* C's fine cells are the region cells pr..pr+3 on each axis */
+        const int ro = (((lane >> 4) + pr) * RF + ((lane >> 2) & 3) + pr) * RF + (lane & 3) + pr;
+        const int no = off[ro + 1] - off[ro];
+        own0[lane] = wave_incl_scan(no) - no;
+      }
       __syncthreads();
       for (int q = wv; q < 27; q += PPC_T / 64) {
         const int np = snp[q];
@@ -676,9 +674,27 @@ __global__ __launch_bounds__(PPC_T) void k_pp_cell(const typename IntOf<IX>::T *
           }
           /* every lane has read its count: the group's last lane advances it */
           if (ri >= 0 && ((m >> lane) >> 1) == 0ull) cnt[ri] = done + __popcll(m);
+          if (q == 13 && w0 == 0 && ri >= 0) {
+            /* C's own particle: its rank among C's particles in staged order */
+            const int rx = ri % RF - pr, ry = (ri / RF) % RF - pr, rz = ri / (RF * RF) - pr;
+            const int k = own0[(rz * 4 + ry) * 4 + rx] + done + __popcll(m & ((1ull << lane) - 1ull)) - c * PPC_T;
+            if (k >= 0 && k < PPC_T) tgt[k] = l;
+          }
         }
       }
       __syncthreads();
+      if (w0 == 0 && act) {
+        p = sp0[13] + tgt[tid];
+        int gf[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const int32_t code = (int32_t)xp[3 * p + a];
+          x1[a] = (double)(pp_fix(IX, C[a], code) - X0[a]) * 0x1p-15;
+          gf[a] = 4 * (C[a] - 1) + (int)(cube_ucode(IX, code) >> sh) + 1;
+          vreal[a] = vdec[(int32_t)vp[3 * p + a] + nvh];
+        }
+        f00 = ((gf[2] - pr - flo[2]) * RF + (gf[1] - pr - flo[1])) * RF + (gf[0] - pr - flo[0]);
+      }
       if (act) {
         /* the particle's (2pr+1)^2 row ranges clipped to [w0, w1), one flat
            loop as in the block kernel */
