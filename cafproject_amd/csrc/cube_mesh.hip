@@ -153,23 +153,29 @@ __device__ __forceinline__ void gcoords(int64_t cg, int nc, int gg[3]) {
   }
 }
 #define FD_T 128
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rl_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 template <int IX>
 __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<IX>::T *__restrict__ xp,
                                                           const int32_t *__restrict__ rhoc,
                                                           const int64_t *__restrict__ cstart, Geo g, float mass,
-                                                          float *__restrict__ rho, double *__restrict__ part) {
+                                                          float *__restrict__ rho, double *__restrict__ part,
+                                                          int heavy_at) {
   __shared__ float acc[64 * FD_T]; /* element q of thread t at q*FD_T + t: conflict free */
   const int nc = g.nc, nf = 4 * nc;
   const int64_t nc3 = (int64_t)nc * nc * nc;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double tsum = 0;
   for (int64_t cg = blockIdx.x * (int64_t)FD_T + t; cg - t < nc3; cg += (int64_t)gridDim.x * FD_T) {
     const bool live = cg < nc3;
     for (int q = 0; q < 64; ++q) acc[q * FD_T + t] = 0.f;
+    int gg[3] = {0, 0, 0}, C[3] = {1, 1, 1}, za = 0, zb = 0;
+    int npc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t p0c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool heavy = false;
     if (live) {
-      int gg[3];
       gcoords(cg, nc, gg);
-      int T[3], C[3];
+      int T[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         T[a] = gg[a] / g.nt;
@@ -189,7 +195,8 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
           dm[a] = C[a] > 1 ? -stride[a] : stride[a] * (nt - 1) + tstride[a] * (T[a] > 0 ? -1 : g.nnt - 1);
       }
       /* the two source z cells in (ilayer, k) order, ilayer = (k-k0) mod 3 */
-      int za = C[2] - 1, zb = C[2];
+      za = C[2] - 1;
+      zb = C[2];
       {
         const int k0 = 2 - g.ncb;
         const int ka = ((za - k0) % 3 + 3) % 3, kb = ((zb - k0) % 3 + 3) % 3;
@@ -199,8 +206,6 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
         }
       }
       /* counts and starts of the 8 source cells, all loads issued up front */
-      int npc[8];
-      int64_t p0c[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int64_t cc = cidC + ((q & 1) ? 0 : dm[0]) + (((q >> 1) & 1) ? 0 : dm[1]) +
@@ -208,6 +213,8 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
         npc[q] = rhoc[cc];
         p0c[q] = cstart[cc];
       }
+      heavy = ((npc[0] + npc[1]) + (npc[2] + npc[3])) + ((npc[4] + npc[5]) + (npc[6] + npc[7])) > heavy_at;
+      if (!heavy)
 #pragma unroll 1
       for (int qz = 0; qz < 2; ++qz)
 #pragma unroll 1
@@ -309,6 +316,80 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
               }
             }
           }
+    }
+    /* a heavy lane's cell is formed by its whole wave: lane f owns fine
+       cell f = (z*4+y)*4+x of it and adds, in the same order and with the
+       same expression as the lane would, the terms of the candidates that
+       64 lanes decode at a time */
+    uint64_t hm = __ballot(live && heavy);
+    while (hm) {
+      const int L = __builtin_ctzll(hm);
+      hm &= hm - 1;
+      const int LC[3] = {rl_i(C[0], L), rl_i(C[1], L), rl_i(C[2], L)};
+      const int Lza = rl_i(za, L), Lzb = rl_i(zb, L);
+      const int fx = lane & 3, fy = (lane >> 2) & 3, fz = lane >> 4;
+      float sacc = 0.f;
+      for (int qi = 0; qi < 8; ++qi) {
+        const int qx = qi & 1, qy = (qi >> 1) & 1, qz = qi >> 2;
+        const int ci[3] = {LC[0] - 1 + qx, LC[1] - 1 + qy, qz ? Lzb : Lza};
+        int np = 0;
+        int64_t p0 = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { /* the owner's counts and starts */
+          const int nq = rl_i(npc[q], L);
+          const int64_t pq = (int64_t)(((uint64_t)(uint32_t)rl_i((int)(p0c[q] >> 32), L) << 32) |
+                                       (uint64_t)(uint32_t)rl_i((int)(p0c[q] & 0xffffffff), L));
+          np = qi == q ? nq : np;
+          p0 = qi == q ? pq : p0;
+        }
+        const int sh = IX == 1 ? 6 : 14;
+        const bool ex = qx == 0, ey = qy == 0, ez = ci[2] == LC[2] - 1;
+        for (int lb = 0; lb < np; lb += 64) {
+          const int l = lb + lane;
+          bool pass = false;
+          float d1v[3] = {0.f, 0.f, 0.f}, d2v[3] = {0.f, 0.f, 0.f};
+          int rp = 0;
+          if (l < np) {
+            int32_t cq[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) cq[a] = (int32_t)xp[3 * (p0 + l) + a];
+            pass = (!ex || (cube_ucode(IX, cq[0]) >> sh) == 3) && (!ey || (cube_ucode(IX, cq[1]) >> sh) == 3) &&
+                   (!ez || (cube_ucode(IX, cq[2]) >> sh) == 3);
+            if (pass) {
+              int r[3];
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                const float off4 = ((float)cube_ucode(IX, cq[a]) + 0.5f) * (IX == 1 ? 0x1p-6f : 0x1p-14f);
+                const float tempx = (float)(4 * (ci[a] - 1)) + off4;
+                const int i1 = (int)floorf(tempx) + 1;
+                d1v[a] = (float)i1 - tempx;
+                d2v[a] = 1.0f - d1v[a];
+                r[a] = i1 - 4 * (LC[a] - 1) - 1;
+              }
+              rp = ((r[0] + 1) & 7) | (((r[1] + 1) & 7) << 3) | (((r[2] + 1) & 7) << 6);
+            }
+          }
+          uint64_t m = __ballot(pass);
+          while (m) { /* candidates in increasing l, read from their lanes */
+            const int i = __builtin_ctzll(m);
+            m &= m - 1;
+            const int ri = rl_i(rp, i);
+            const int cx = fx - ((ri & 7) - 1), cy = fy - (((ri >> 3) & 7) - 1), cz = fz - (((ri >> 6) & 7) - 1);
+            const float ax0 = rl_f(d1v[0], i), ax1 = rl_f(d2v[0], i), ay0 = rl_f(d1v[1], i), ay1 = rl_f(d2v[1], i);
+            const float az0 = rl_f(d1v[2], i), az1 = rl_f(d2v[2], i);
+            if ((unsigned)cx <= 1u && (unsigned)cy <= 1u && (unsigned)cz <= 1u) {
+              const float wx = cx ? ax1 : ax0, wy = cy ? ay1 : ay0, wz = cz ? az1 : az0;
+              sacc = sacc + wx * wy * wz * mass;
+            }
+          }
+        }
+      }
+      acc[lane * FD_T + wv * 64 + L] = sacc;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (live) {
       const int64_t f0x = 4 * (int64_t)gg[0], f0y = 4 * (int64_t)gg[1], f0z = 4 * (int64_t)gg[2];
       for (int z = 0; z < 4; ++z)
         for (int y = 0; y < 4; ++y) {
@@ -339,37 +420,75 @@ __global__ __launch_bounds__(FD_T) void k_fine_deposit_cc(const typename IntOf<I
  * m0-1..m0+2 per axis, walked in the reference order (ilayer=k mod 3, k,
  * j, i, l), with the 8 sums in registers.  A particle that misses a cell
  * adds +0.0, which leaves a sum of non-negative terms unchanged.
+ *
+ * The 16 source rows (k, j) are gathered in that order.  A lane whose
+ * block's sources pass `heavy_at` particles (a clustered region) stops at
+ * a row boundary, and its whole wave finishes the block: 64 lanes form 64
+ * particles' 8 terms at once (the same expression), and 8 lanes add them
+ * to the partial sums in particle order -- the same additions in the same
+ * order, so the same bits, without one lane walking thousands of particles
+ * while the other 63 wait.  The owner adds the 8 sums to its total in the
+ * same place of its sequence as a light block's.
  */
+template <int IX>
+__device__ __forceinline__ void cd_weights(const int ci[3], const int m0[3], const int32_t code[3], float w0[3],
+                                           float w1[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float tempx = coarse_tempx<IX>(ci[a] - 1, code[a]);
+    const int i1 = (int)floorf(tempx) + 1;
+    const float d1 = (float)i1 - tempx, d2 = 1.0f - d1;
+    const int r = i1 - m0[a]; /* corner 1 relative to the block */
+    w0[a] = r == 0 ? d1 : (r == -1 ? d2 : 0.f);
+    w1[a] = r == 1 ? d1 : (r == 0 ? d2 : 0.f);
+  }
+}
+
 template <int IX>
 __global__ __launch_bounds__(MT) void k_coarse_deposit_b2(const typename IntOf<IX>::T *__restrict__ xp,
                                                           const int32_t *__restrict__ rhoc,
                                                           const int64_t *__restrict__ cstart, Geo g, float mass,
-                                                          float *__restrict__ r3, double *__restrict__ part) {
+                                                          float *__restrict__ r3, double *__restrict__ part,
+                                                          int heavy_at) {
   const int nc = g.nc, nb = nc / 2;
   const int64_t nb3 = (int64_t)nb * nb * nb;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ float cw[MT / 64][64 * 9]; /* a wave's 64 particles x 8 terms, rows of 9: no bank conflicts */
+  __shared__ float sfin[MT / 64][8];
   double tsum = 0;
-  for (int64_t b = blockIdx.x * (int64_t)MT + threadIdx.x; b < nb3; b += (int64_t)gridDim.x * MT) {
-    const int gg[3] = {2 * (int)(b % nb), 2 * (int)((b / nb) % nb), 2 * (int)(b / ((int64_t)nb * nb))};
-    int T[3], m0[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      T[a] = gg[a] / g.nt;
-      m0[a] = gg[a] - T[a] * g.nt + 1;
-    }
-    int zs[4] = {m0[2] - 1, m0[2], m0[2] + 1, m0[2] + 2};
-    /* order by (k mod 3, k): insertion sort of 4 */
-    for (int a = 1; a < 4; ++a) {
-      const int x = zs[a];
-      int bb = a - 1;
-      while (bb >= 0 && ((zs[bb] % 3) > (x % 3) || ((zs[bb] % 3) == (x % 3) && zs[bb] > x))) {
-        zs[bb + 1] = zs[bb];
-        --bb;
-      }
-      zs[bb + 1] = x;
-    }
+  /* the loop is uniform over a wave: lane l takes block b0 + l */
+  for (int64_t b0 = blockIdx.x * (int64_t)MT + wv * 64; b0 < nb3; b0 += (int64_t)gridDim.x * MT) {
+    const int64_t b = b0 + lane;
+    const bool live = b < nb3;
+    int gg[3] = {0, 0, 0}, T[3] = {0, 0, 0}, m0[3] = {1, 1, 1}, zs[4] = {0, 0, 0, 0};
     float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int qz = 0; qz < 4; ++qz)
-      for (int sy = m0[1] - 1; sy <= m0[1] + 2; ++sy) {
+    int rrow = 16; /* first source row the lane has not gathered */
+    if (live) {
+      gg[0] = 2 * (int)(b % nb);
+      gg[1] = 2 * (int)((b / nb) % nb);
+      gg[2] = 2 * (int)(b / ((int64_t)nb * nb));
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        T[a] = gg[a] / g.nt;
+        m0[a] = gg[a] - T[a] * g.nt + 1;
+      }
+      zs[0] = m0[2] - 1;
+      zs[1] = m0[2];
+      zs[2] = m0[2] + 1;
+      zs[3] = m0[2] + 2;
+      /* order by (k mod 3, k): insertion sort of 4 */
+      for (int a = 1; a < 4; ++a) {
+        const int x = zs[a];
+        int bb = a - 1;
+        while (bb >= 0 && ((zs[bb] % 3) > (x % 3) || ((zs[bb] % 3) == (x % 3) && zs[bb] > x))) {
+          zs[bb + 1] = zs[bb];
+          --bb;
+        }
+        zs[bb + 1] = x;
+      }
+      int cum = 0;
+      for (int row = 0; row < 16; ++row) {
+        const int qz = row >> 2, sy = m0[1] - 1 + (row & 3);
         /* the row's 4 source cells: counts and starts loaded together */
         int npr[4];
         int64_t p0r[4];
@@ -378,6 +497,11 @@ __global__ __launch_bounds__(MT) void k_coarse_deposit_b2(const typename IntOf<I
           const int64_t cc = cid_global(g, T[0] * g.nt + m0[0] - 2 + u, T[1] * g.nt + sy - 1, T[2] * g.nt + zs[qz] - 1);
           npr[u] = rhoc[cc];
           p0r[u] = cstart[cc];
+        }
+        cum += (npr[0] + npr[1]) + (npr[2] + npr[3]);
+        if (cum > heavy_at) {
+          rrow = row;
+          break;
         }
 #pragma unroll 1
         for (int sx = m0[0] - 1; sx <= m0[0] + 2; ++sx) {
@@ -392,41 +516,100 @@ __global__ __launch_bounds__(MT) void k_coarse_deposit_b2(const typename IntOf<I
           }
           /* four particles' codes in flight at once (the last one repeated) */
           for (int l0 = 0; l0 < np; l0 += 4) {
-          int32_t code4[4][3];
+            int32_t code4[4][3];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int64_t q = p0 + min(l0 + u, np - 1);
+            for (int u = 0; u < 4; ++u) {
+              const int64_t q = p0 + min(l0 + u, np - 1);
 #pragma unroll
-            for (int a = 0; a < 3; ++a) code4[u][a] = (int32_t)xp[3 * q + a];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (l0 + u >= np) break;
-            float w0[3], w1[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const int32_t code = code4[u][a];
-              const float tempx = coarse_tempx<IX>(ci[a] - 1, code);
-              const int i1 = (int)floorf(tempx) + 1;
-              const float d1 = (float)i1 - tempx, d2 = 1.0f - d1;
-              const int r = i1 - m0[a]; /* corner 1 relative to the block */
-              w0[a] = r == 0 ? d1 : (r == -1 ? d2 : 0.f);
-              w1[a] = r == 1 ? d1 : (r == 0 ? d2 : 0.f);
+              for (int a = 0; a < 3; ++a) code4[u][a] = (int32_t)xp[3 * q + a];
             }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-              const float wx = (q & 1) ? w1[0] : w0[0], wy = (q & 2) ? w1[1] : w0[1], wz = (q & 4) ? w1[2] : w0[2];
-              s[q] = s[q] + wx * wy * wz * mass;
+            for (int u = 0; u < 4; ++u) {
+              if (l0 + u >= np) break;
+              float w0[3], w1[3];
+              cd_weights<IX>(ci, m0, code4[u], w0, w1);
+#pragma unroll
+              for (int q = 0; q < 8; ++q) {
+                const float wx = (q & 1) ? w1[0] : w0[0], wy = (q & 2) ? w1[1] : w0[1], wz = (q & 4) ? w1[2] : w0[2];
+                s[q] = s[q] + wx * wy * wz * mass;
+              }
             }
-          }
           }
         }
       }
+    }
+    /* the wave finishes each heavy lane's block from its row rrow on */
+    uint64_t hm = __ballot(live && rrow < 16);
+    while (hm) {
+      const int L = __builtin_ctzll(hm);
+      hm &= hm - 1;
+      const int Lgg[3] = {rl_i(gg[0], L), rl_i(gg[1], L), rl_i(gg[2], L)};
+      const int LT[3] = {rl_i(T[0], L), rl_i(T[1], L), rl_i(T[2], L)};
+      const int Lm0[3] = {rl_i(m0[0], L), rl_i(m0[1], L), rl_i(m0[2], L)};
+      const int Lzs[4] = {rl_i(zs[0], L), rl_i(zs[1], L), rl_i(zs[2], L), rl_i(zs[3], L)};
+      const int Lrow = rl_i(rrow, L);
+      float acc = 0.f; /* lane q < 8: sum q, from the owner's partial sum */
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int x = gg[0] + (q & 1), y = gg[1] + ((q >> 1) & 1), z = gg[2] + ((q >> 2) & 1);
-      r3[((int64_t)z * nc + y) * nc + x] = s[q];
-      tsum += (double)s[q];
+      for (int q = 0; q < 8; ++q) {
+        const float v = rl_f(s[q], L);
+        acc = lane == q ? v : acc;
+      }
+      for (int row = Lrow; row < 16; ++row) {
+        const int qz = row >> 2, sy = Lm0[1] - 1 + (row & 3);
+        for (int u = 0; u < 4; ++u) {
+          const int sx = Lm0[0] - 1 + u;
+          const int64_t cc = cid_global(g, LT[0] * g.nt + sx - 1, LT[1] * g.nt + sy - 1, LT[2] * g.nt + Lzs[qz] - 1);
+          const int np = rhoc[cc];
+          const int64_t p0 = cstart[cc];
+          const int ci[3] = {sx, sy, Lzs[qz]};
+          for (int base = 0; base < np; base += 64) {
+            const int l = base + lane;
+            if (l < np) {
+              int32_t code[3];
+#pragma unroll
+              for (int a = 0; a < 3; ++a) code[a] = (int32_t)xp[3 * (p0 + l) + a];
+              float w0[3], w1[3];
+              cd_weights<IX>(ci, Lm0, code, w0, w1);
+#pragma unroll
+              for (int q = 0; q < 8; ++q) {
+                const float wx = (q & 1) ? w1[0] : w0[0], wy = (q & 2) ? w1[1] : w0[1], wz = (q & 4) ? w1[2] : w0[2];
+                cw[wv][lane * 9 + q] = wx * wy * wz * mass;
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < 8) {
+              const int m = min(64, np - base);
+              for (int i = 0; i < m; ++i) acc = acc + cw[wv][i * 9 + lane];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          }
+        }
+      }
+      if (lane < 8) {
+        const int x = Lgg[0] + (lane & 1), y = Lgg[1] + ((lane >> 1) & 1), z = Lgg[2] + ((lane >> 2) & 1);
+        r3[((int64_t)z * nc + y) * nc + x] = acc;
+        sfin[wv][lane] = acc;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane == L)
+        for (int q = 0; q < 8; ++q) tsum += (double)sfin[wv][q];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (live && rrow == 16) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int x = gg[0] + (q & 1), y = gg[1] + ((q >> 1) & 1), z = gg[2] + ((q >> 2) & 1);
+        r3[((int64_t)z * nc + y) * nc + x] = s[q];
+        tsum += (double)s[q];
+      }
     }
   }
   __shared__ double red[MT];
@@ -441,8 +624,13 @@ __global__ __launch_bounds__(MT) void k_coarse_deposit_b2(const typename IntOf<I
 
 #define DEP_BLOCKS 2048
 template <int IX> static int fine_dep(cube_ctx *c, float mass, double *t) {
+  /* a coarse cell whose 8 source cells hold more particles than this is
+     formed by its lane's whole wave (CUBE_FD_HEAVY) */
+  int heavy_at = 512;
+  if (const char *e = getenv("CUBE_FD_HEAVY"); e && *e) heavy_at = std::max(0, atoi(e));
   hipLaunchKernelGGL(k_fine_deposit_cc<IX>, dim3(DEP_BLOCKS), dim3(FD_T), 0, c->stream,
-                     (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_f, c->dred);
+                     (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_f, c->dred,
+                     heavy_at);
   CUBE_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum_parts1, dim3(1), dim3(64), 0, c->stream, c->dred, DEP_BLOCKS,
                      c->dred + CUBE_DRED_FINE_SUM);
@@ -460,9 +648,14 @@ template <int IX> static int coarse_dep(cube_ctx *c, float mass, double *t) {
   /* partial sums in the upper half of dred, apart from the fine deposit's,
      so that the two deposits may run at the same time */
   double *part = c->dred + 4096;
+  /* source particles a lane gathers alone before its wave takes the rest
+     of its block (CUBE_CD_HEAVY) */
+  int heavy_at = 1024;
+  if (const char *e = getenv("CUBE_CD_HEAVY"); e && *e) heavy_at = std::max(0, atoi(e));
   if (c->nt % 2 == 0)
     hipLaunchKernelGGL(k_coarse_deposit_b2<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
-                       (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, part);
+                       (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, part,
+                       heavy_at);
   else
     hipLaunchKernelGGL(k_coarse_deposit<IX>, dim3(DEP_BLOCKS), dim3(MT), 0, c->stream,
                        (const typename IntOf<IX>::T *)c->xp, c->rhoc, c->cstart, geo_of(c), mass, c->rho_c, part);

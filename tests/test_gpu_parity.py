@@ -79,7 +79,14 @@ DEP_CASES = CASES + [dict(nc=20, nnt=2), dict(nc=18, nnt=2, izipx=1, izipv=2)]
 
 
 @pytest.mark.parametrize("case", DEP_CASES, ids=_ids)
-def test_fine_and_coarse_deposit_bit_exact(case):
+@pytest.mark.parametrize("heavy", [None, "0"])
+def test_fine_and_coarse_deposit_bit_exact(case, heavy, monkeypatch):
+    """Deposits bit for bit; with CUBE_CD_HEAVY=0 and CUBE_FD_HEAVY=0 every
+    coarse 2^3 block and every fine-deposit coarse cell is formed by its
+    whole wave (the path of clustered regions)."""
+    if heavy is not None:
+        monkeypatch.setenv("CUBE_CD_HEAVY", heavy)
+        monkeypatch.setenv("CUBE_FD_HEAVY", heavy)
     st, oc, g = _pair(case)
     m = st["mass_p"]
     tf = g.fine_deposit(m)
@@ -172,6 +179,49 @@ def test_pp_bit_exact(case, stage_max, cell_cap, monkeypatch):
     assert f_g == f_o
 
 
+def _clump_state(nc=16, seed=5, nclump=6000, nrest=10000):
+    """A collapsed clump (Gaussian, sigma 0.7 coarse cells) in a uniform
+    background, as continuous positions and velocities."""
+    rng = np.random.default_rng(seed)
+    clump = np.array([7.3, 8.1, 8.6]) + rng.normal(0, 0.7, (nclump, 3))
+    rest = rng.uniform(0, nc, (nrest, 3))
+    pos = np.concatenate([clump, rest]) % nc
+    vel = rng.normal(0, 0.3, (nclump + nrest, 3)).astype(np.float32)
+    return pos, vel
+
+
+@pytest.mark.parametrize("heavy", ["0", "40", "2048", "100000000"])
+def test_deposits_clustered_bit_exact(heavy, monkeypatch):
+    """The clump state: fine and coarse meshes bit for bit against the
+    oracle, with the clustered coarse blocks and fine-deposit cells formed
+    by their waves always (0), often (40), at 2048 source particles and
+    never; the totals are the same numbers whichever lanes gathered."""
+    from cafproject_amd.cube import Cube
+    nc, nnt = 16, 1
+    pos, vel = _clump_state(nc)
+    sig = 0.3
+    g = Cube(nc, nnt)
+    g.set_state_xv(pos, vel, sig)
+    oc = O.OracleCube(nc, nnt)
+    oc.state_from_xv(pos, vel.astype(np.float64), None, sig)
+    oc.buffer_grid(); oc.buffer_x(); oc.buffer_v()
+    m = float(np.float32((4 * nc) ** 3 / len(pos)))
+    tc_default = g.coarse_deposit(m)
+    tf_default = g.fine_deposit(m)
+    monkeypatch.setenv("CUBE_CD_HEAVY", heavy)
+    monkeypatch.setenv("CUBE_FD_HEAVY", heavy)
+    tc = g.coarse_deposit(m)
+    r3_o, sc = oc.coarse_deposit(m)
+    assert np.array_equal(g.get_field("rho_c"), r3_o)
+    assert tc == tc_default
+    assert abs(tc / sc - 1) < 1e-12
+    assert g.fine_deposit(m) == tf_default
+    rho, _ = oc.fine_deposit(1, 1, 1, m)
+    nfb, nft = oc.nfb, oc.nft
+    tiles = {(1, 1, 1): dict(rho=rho[nfb - 1:nfb + nft + 1, nfb - 1:nfb + nft + 1, nfb - 1:nfb + nft + 1])}
+    assert np.array_equal(g.get_field("rho_f"), global_from_tiles(tiles, nc, nnt, key="rho"))
+
+
 @pytest.mark.parametrize("stage_max,cell_cap", [(None, None), ("0", None), (None, "300")])
 def test_pp_clustered_bit_exact(stage_max, cell_cap, monkeypatch):
     """A collapsed clump (6000 particles, Gaussian with sigma 0.7 coarse
@@ -185,11 +235,7 @@ def test_pp_clustered_bit_exact(stage_max, cell_cap, monkeypatch):
     if cell_cap is not None:
         monkeypatch.setenv("CUBE_PP_CELL_CAP", cell_cap)
     nc, nnt = 16, 1
-    rng = np.random.default_rng(5)
-    clump = np.array([7.3, 8.1, 8.6]) + rng.normal(0, 0.7, (6000, 3))
-    rest = rng.uniform(0, nc, (10000, 3))
-    pos = np.concatenate([clump, rest]) % nc
-    vel = rng.normal(0, 0.3, (16000, 3)).astype(np.float32)
+    pos, vel = _clump_state(nc)
     sig = 0.3
     g = Cube(nc, nnt, pp_on=True)
     g.set_state_xv(pos, vel, sig)
