@@ -287,13 +287,64 @@ def batched_gradients(iters=3):
 
 
 Z_BINS = (49.0, 20.0, 10.0, 5.0, 3.0, 2.0, 1.0, 0.5, 0.25, 0.0)
+Z_PROBES = (1.0, 0.0)
+
+
+def pair_count(st, nc, nnt):
+    """PP pair visits of a state: the sum over particles of the particles in
+    their 27 neighbouring fine cells (the particle itself included), the
+    (2*pp_range+1)^3 walk of W/ext_pp_force.f90:94-115 at pp_range=1; counted
+    on the GPU with torch from the cell-sorted codes."""
+    import torch
+    dev = torch.device("cuda")
+    nt, nf = nc // nnt, 4 * nc
+    rhoc = torch.from_numpy(st["rhoc"]).to(dev)
+    cell = torch.repeat_interleave(torch.arange(rhoc.numel(), device=dev, dtype=torch.int64), rhoc.long())
+    loc = [cell % nt, (cell // nt) % nt, (cell // nt ** 2) % nt]
+    t = cell // nt ** 3
+    tile = [t % nnt, (t // nnt) % nnt, t // nnt ** 2]
+    del cell, t
+    codes = torch.from_numpy(st["xp"].view(np.uint8) if st["xp"].dtype == np.int8 else
+                             (st["xp"].view(np.uint16) >> 8).astype(np.uint8)).to(dev).long()
+    f = [4 * (tile[a] * nt + loc[a]) + (codes[:, a] >> 6) for a in range(3)]
+    del loc, tile, codes
+    n = torch.bincount((f[2] * nf + f[1]) * nf + f[0], minlength=nf ** 3).view(nf, nf, nf).to(torch.int32)
+    del f
+    s = n.clone()
+    for ax in (2, 1, 0):
+        s = s + torch.roll(s, 1, ax) + torch.roll(s, -1, ax)
+    pairs = int((n.long() * s.long()).sum().item())
+    del n, s
+    torch.cuda.empty_cache()
+    return pairs
+
+
+def alone(st, nc, nnt, izipx, a_mid, dt, sigma, mass, reps=3):
+    """PP and both deposits on a copy of the state in a second context
+    without Green's functions (so nothing runs beside PP): fastest of reps."""
+    from cafproject_amd.cube import Cube
+    g2 = Cube(nc, nnt, izipx=izipx, izipv=2, has_pid=False, pp_on=True, pp_range=1, rsoft=0.3)
+    best = dict(pp=1e30, fine_deposit=1e30, coarse_deposit=1e30)
+    for _ in range(reps):
+        g2.set_state(st["xp"], st["vp"], None, st["rhoc"], st["vfield"])
+        g2.sync()
+        for k, fn in (("pp", lambda: g2.ext_pp_force(a_mid, dt, sigma, mass)),
+                      ("fine_deposit", lambda: g2.fine_deposit(mass)), ("coarse_deposit", lambda: g2.coarse_deposit(mass))):
+            t = time.perf_counter()
+            fn()
+            g2.sync()
+            best[k] = min(best[k], 1e3 * (time.perf_counter() - t))
+    del g2
+    return {k: round(v, 2) for k, v in best.items()}
 
 
 def to_z0(nc, nnt, box, izipx, pp_on, max_steps=3000):
     """The whole run of the headline workload, z=49 -> 0 with the
     reference's step criterion (W/main.f90:88-145): every step's wall time
     and stage times (PP, deposits, drift, kicks) binned by redshift, the
-    particle count after every step, and P(k) of the final state."""
+    particle count after every step, P(k) of the final state, and at z = 1
+    and z = 0 the state's PP pair count with PP and the deposits timed alone
+    (outside the run's clock)."""
     from cafproject_amd.cube import Cube
     g = Cube(nc, nnt, izipx=izipx, izipv=2, has_pid=False, pp_on=pp_on, pp_range=1, rsoft=0.3)
     g.init_kernels()
@@ -305,7 +356,9 @@ def to_z0(nc, nnt, box, izipx, pp_on, max_steps=3000):
     g.profile(True)
     g.timers(reset=True)
     rows = []
+    probes = []
     t0 = time.perf_counter()
+    zp = list(Z_PROBES)
     while not step.clock.final_step and len(rows) < max_steps:
         a0 = float(step.clock.a)
         t = time.perf_counter()
@@ -319,6 +372,19 @@ def to_z0(nc, nnt, box, izipx, pp_on, max_steps=3000):
         rows.append(dict(z=1 / a1 - 1, ms=ms, pp=tm.get("pp", 0.0), drift=tm["drift"],
                          fine_deposit=tm["fine_deposit"], coarse_deposit=tm["coarse_deposit"],
                          kick=tm["fine_kick"] + tm["coarse_kick"], da=a1 - a0))
+        if zp and (1 / a1 - 1 <= zp[0] + 1e-6):
+            # PP and the deposits alone on this state, and its pair count
+            # (outside the run's clock)
+            tp = time.perf_counter()
+            st_h = g.get_state()
+            pairs = pair_count(st_h, nc, nnt)
+            al = alone(st_h, nc, nnt, izipx, float(step.clock.a), float(step.clock.dt), step.sigma, mass)
+            del st_h
+            probes.append(dict(z=round(1 / a1 - 1, 4), pairs=pairs, pairs_per_particle=round(pairs / n, 2),
+                               alone_ms=al, pp_pairs_per_s=pairs / (al["pp"] * 1e-3)))
+            log(f"[bench] z0 run probe at z={1 / a1 - 1:.3f}: {pairs / n:.1f} pairs per particle, alone {al}")
+            zp.pop(0)
+            t0 += time.perf_counter() - tp
         if len(rows) % 10 == 0 or step.clock.final_step:
             log(f"[bench] z0 run step {len(rows)} a={a1:.4f} z={1 / a1 - 1:.3f} {ms:.1f} ms "
                 f"(pp {rows[-1]['pp']:.1f}, deposits {tm['fine_deposit']:.1f}/{tm['coarse_deposit']:.1f})")
@@ -346,7 +412,7 @@ def to_z0(nc, nnt, box, izipx, pp_on, max_steps=3000):
                                f"nc={nc}, {nnt}^3 tiles, int{8 * izipx} x / int16 v, PP {'on' if pp_on else 'off'}, "
                                f"Zel'dovich EH no-wiggle, seed 42", "particles": n},
         "a_final": 1 / (1 + rows[-1]["z"]) if rows else None,
-        "particles_conserved": True, "by_redshift": bins,
+        "particles_conserved": True, "by_redshift": bins, "alone_at": probes,
         "delta2_z0": {"k_h_per_mpc": list(pk), "delta2": list(pk.values())},
     }
 
