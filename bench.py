@@ -33,6 +33,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-updates/s at 512^3 (PM+PP step); batched adjoint grads/s vs plain"
 HBM_PEAK = 8.0e12
+FP64_PEAK_TF = 78.6  # MI355X FP64, TFLOP/s
+FLOP_PER_PAIR = 33
 
 
 def log(*a):
@@ -472,6 +474,18 @@ def main():
         step()
         log(f"[bench] warmup step {i} {1e3 * (time.perf_counter() - t):.1f} ms a={float(step.clock.a):.5f}")
     g.sync()
+
+    def pairs_now():
+        """PP pair visits of the current state (outside the timed region)."""
+        if not pp_on:
+            return None
+        try:
+            return pair_count(g.get_state(), nc, nnt)
+        except Exception as e:  # the count must not hide the measurement
+            log(f"[bench] pair count skipped: {e}")
+            return None
+
+    pairs0 = pairs_now()
     g.profile(True)
     g.timers(reset=True)
     t0 = time.perf_counter()
@@ -481,6 +495,7 @@ def main():
     elapsed = time.perf_counter() - t0
     tm = g.timers()
     g.profile(False)
+    pairs1 = pairs_now()
     ms_step = 1e3 * elapsed / args.steps
     value = np_ * args.steps / elapsed
     bx, bv = 3 * izipx, 3 * izipv
@@ -518,6 +533,18 @@ def main():
 
     dom = max(single, key=lambda k: per_launch[k]["ms"])
     roofline = kernel_entry(dom)
+    if dom == "pp" and pairs0 and pairs1:
+        # PP is FP64 pair arithmetic: the reference's pair force counts 33
+        # FP64 operations (W/ext_pp_force.f90:103-110: xvec21 3, |r|^2 5 and
+        # its sqrt 1, rcut = r/16 and the cutoff polynomial 10, r^3 2, the 3
+        # quotients, mass and pcut products 6, the 3 sums; the position
+        # decode and the *0 of soft pairs not counted); pairs per step from
+        # the states before and after the window
+        pairs = 0.5 * (pairs0 + pairs1)
+        tf = pairs * FLOP_PER_PAIR / (per_launch["pp"]["ms"] * 1e-3) / 1e12
+        roofline["fp64"] = {"achieved": tf, "peak": FP64_PEAK_TF, "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TF,
+                            "flop_per_pair": FLOP_PER_PAIR, "pairs_per_step": [pairs0, pairs1],
+                            "pairs_per_s": pairs / (per_launch["pp"]["ms"] * 1e-3)}
     alone = [k for k in single if k not in CONCURRENT and k != dom]
     if alone:
         roofline["secondary"] = kernel_entry(max(alone, key=lambda k: per_launch[k]["ms"]))

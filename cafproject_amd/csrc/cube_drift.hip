@@ -871,7 +871,7 @@ int cube_drift(cube_ctx *c, float dt_old, float dt, float vz_max, float sigma_vi
   P.agg_cap = AGG_CAP;
   /* a lower cap sends more rows down the all-global slot path (used by the
      tests to cover both paths in one run) */
-  if (const char *e = getenv("CUBE_DRIFT_AGG_CAP")) P.agg_cap = std::min(AGG_CAP, atoi(e));
+  if (const char *e = getenv("CUBE_DRIFT_AGG_CAP"); e && *e) P.agg_cap = std::min(AGG_CAP, atoi(e));
   const double venc = cube_vconst(sigma_vi);
   int64_t nxtra = 0;
   unsigned long long hc[8];

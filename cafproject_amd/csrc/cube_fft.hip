@@ -337,13 +337,9 @@ template <int N> struct RC {
   static constexpr int R2 = N / R1;
   static constexpr int CB = 16;
   static constexpr int NT = CB * R1;
-  /* LDS transpose buffer S[c][q][t]: c stride = 4 (mod 64), q stride
-     R1+2 (= 2 mod 4).  The compiler pairs the transpose's accesses into
-     ds_write2_b32 (q, q+1) and ds_read2_b32 (p, p+1), and the LDS serves
-     such an instruction a half-wave (16 c x 2 t) at a time: the four dword
-     offsets {0, 1, SQ, SQ+1} then fall in distinct residues mod 4 while 4c
-     covers the multiples of 4, so no two accesses share a bank */
-  static constexpr int SQ = R1 + 2;
+  /* LDS transpose buffer S[c][q][t]: q stride R1+1, c stride = 4 (mod 64)
+     so both the row writes and the column reads of a wave hit distinct banks */
+  static constexpr int SQ = R1 + 1;
   static constexpr int SC0 = R2 * SQ;
   static constexpr int SC = SC0 + ((4 - SC0) % 64 + 64) % 64;
 };
