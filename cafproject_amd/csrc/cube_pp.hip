@@ -742,7 +742,7 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
   /* one division per pair (quot3); CUBE_PP_FASTDIV=0 takes the three
      divisions instead (the same bits) */
   const char *fd = getenv("CUBE_PP_FASTDIV");
-  const int fdiv = !(fd && atoi(fd) == 0);
+  const int fdiv = !(fd && *fd && atoi(fd) == 0);
   if (c->nt % 4 == 0 && pr >= 0 && pr <= 4) {
     const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
     const size_t table = (((size_t)nreg + 1) * 2 + 7) / 8 * 8; /* uint16 per fine cell, plus cur[-1] */
@@ -753,10 +753,15 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
        (under 47 KB at pr = 4) */
     const int RC = 4 + 2 * pr, nregc = RC * RC * RC;
     int cap = 4096;
-    /* lower values send more blocks down the cell path and stage its
-       regions in more windows (used by the tests to cover every path) */
-    if (const char *e = getenv("CUBE_PP_STAGE_MAX")) maxs = std::max(0, std::min(maxs, atoi(e)));
-    if (const char *e = getenv("CUBE_PP_CELL_CAP")) cap = std::max(1, std::min(cap, atoi(e)));
+    /* other values move the split between the two paths and the cell
+       path's window (small ones are used by the tests to cover every path);
+       a window is bounded by the LDS one workgroup may take */
+    int ldsmax = 65536;
+    if (hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+      ldsmax = 65536;
+    if (const char *e = getenv("CUBE_PP_STAGE_MAX"); e && *e) maxs = std::max(0, std::min(maxs, atoi(e)));
+    if (const char *e = getenv("CUBE_PP_CELL_CAP"); e && *e)
+      cap = std::max(1, std::min((ldsmax - 1024 - (2 * nregc + 1) * 4) / 8, atoi(e)));
     const int64_t nb = c->nt / 4, nblk = nb * nb * nb * c->ntiles;
     unsigned long long *dcount = (unsigned long long *)c->ured + 24, *claim = (unsigned long long *)c->ured + 25;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, dcount, 2);
