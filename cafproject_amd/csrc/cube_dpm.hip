@@ -2,10 +2,15 @@
  * cube_dpm.hip -- differentiable particle-mesh forward model and its
  * hand-written adjoint (include/cube_dpm.h).  No reference counterpart.
  *
- * Positions are FP32 in mesh cells of the periodic nf^3 mesh, particles
+ * A particle is held as its FP32 displacement d (in mesh cells) from its
+ * lattice point q = (nf/ng) * (i mod ng, i/ng mod ng, i/ng^2); particles
  * keep their index for the whole run (no sorting), so the adjoint state is
- * per particle.  CIC convention: a particle at x touches mesh points
- * floor(x) and floor(x)+1 per axis with weights 1-f and f (f = x-floor(x)).
+ * per particle.  CIC convention: a particle at x = q + d touches mesh points
+ * floor(x) and floor(x)+1 per axis (mod nf) with weights 1-f and f, formed
+ * as q + floor(d) and f = d - floor(d): exact, and resolved to the
+ * displacement's ulp rather than to the ulp of a position near nf, so far
+ * fewer particles than with FP32 positions sit within rounding of a cell
+ * edge, where the weights' slope (and so the gradient) jumps.
  *
  * Forward operators and their transposes:
  *   deposit  rho_c = sum_i W(x_i,c)          ^T: lam -> sum_c lam_c W(x_i,c)
@@ -89,14 +94,19 @@ struct Cic {
   int i[3][2];   /* mesh indices per axis (periodic) */
   float w[3][2]; /* weights */
 };
-__device__ static inline Cic cic_of(const float *x, int n) {
+/* the lattice point of particle q, in mesh cells: r*(q mod ng, ...) */
+struct Lat {
+  int ng, r;
+};
+__device__ static inline Cic cic_of(const float *dsp, int64_t q, Lat L, int n) {
+  const int l[3] = {(int)(q % L.ng), (int)((q / L.ng) % L.ng), (int)(q / ((int64_t)L.ng * L.ng))};
   Cic c;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float fl = floorf(x[a]);
-    const float f = x[a] - fl;
-    int i0 = (int)fl;
-    i0 = ((i0 % n) + n) % n; /* always a valid mesh index, whatever x holds */
+    const float fl = floorf(dsp[a]);
+    const float f = dsp[a] - fl;
+    int i0 = L.r * l[a] + (int)fl;
+    i0 = ((i0 % n) + n) % n; /* always a valid mesh index, whatever d holds */
     c.i[a][0] = i0;
     c.i[a][1] = i0 + 1 == n ? 0 : i0 + 1;
     c.w[a][0] = 1.f - f;
@@ -114,15 +124,15 @@ __global__ void k_dpm_ic_particles(const float *psi, int ng, int nf, float cx, f
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float s = psi[a * n3 + q] * scale * r;
-      x[3 * q + a] = wrapf((float)l[a] * r + cx * s, (float)nf);
+      x[3 * q + a] = cx * s; /* displacement from the lattice point l*r */
       p[3 * q + a] = cp * s;
     }
   }
 }
 
-__global__ void k_dpm_deposit(const float *x, int64_t np, int n, float *rho) {
+__global__ void k_dpm_deposit(const float *x, int64_t np, Lat L, int n, float *rho) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    const Cic c = cic_of(x + 3 * q, n);
+    const Cic c = cic_of(x + 3 * q, q, L, n);
 #pragma unroll
     for (int z = 0; z < 2; ++z)
 #pragma unroll
@@ -135,9 +145,9 @@ __global__ void k_dpm_deposit(const float *x, int64_t np, int n, float *rho) {
 }
 
 /* p += coef * sum_c W F_c (F interleaved, 3 per mesh point) */
-__global__ void k_dpm_kick(const float *x, float *p, int64_t np, const float *F, int n, float coef) {
+__global__ void k_dpm_kick(const float *x, float *p, int64_t np, const float *F, Lat L, int n, float coef) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    const Cic c = cic_of(x + 3 * q, n);
+    const Cic c = cic_of(x + 3 * q, q, L, n);
     float f[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int z = 0; z < 2; ++z)
@@ -156,9 +166,18 @@ __global__ void k_dpm_kick(const float *x, float *p, int64_t np, const float *F,
   }
 }
 
-__global__ void k_dpm_drift(float *x, const float *p, int64_t np, int n, float coef) {
+/* the displacement needs no wrap: the mesh index is taken mod nf */
+__global__ void k_dpm_drift(float *x, const float *p, int64_t np, float coef) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < 3 * np; q += (int64_t)gridDim.x * blockDim.x)
-    x[q] = wrapf(x[q] + coef * p[q], (float)n);
+    x[q] = x[q] + coef * p[q];
+}
+/* positions q + d wrapped into [0, nf), for cube_dpm_state */
+__global__ void k_dpm_positions(const float *x, int64_t np, Lat L, int n, float *out) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    const int l[3] = {(int)(q % L.ng), (int)((q / L.ng) % L.ng), (int)(q / ((int64_t)L.ng * L.ng))};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[3 * q + a] = wrapf((float)(L.r * l[a]) + x[3 * q + a], (float)n);
+  }
 }
 
 __global__ void k_dpm_delta(const float *rho, int64_t n3, float inv_nbar, float *delta) {
@@ -167,10 +186,10 @@ __global__ void k_dpm_delta(const float *rho, int64_t n3, float inv_nbar, float 
 }
 
 /* S_d[c] += W(x_i,c) g_i,d  (SoA meshes) */
-__global__ void k_dpm_scatter3(const float *x, const float *g, int64_t np, int n, float *S) {
+__global__ void k_dpm_scatter3(const float *x, const float *g, int64_t np, Lat L, int n, float *S) {
   const int64_t n3 = (int64_t)n * n * n;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    const Cic c = cic_of(x + 3 * q, n);
+    const Cic c = cic_of(x + 3 * q, q, L, n);
     const float gv[3] = {g[3 * q], g[3 * q + 1], g[3 * q + 2]};
 #pragma unroll
     for (int z = 0; z < 2; ++z)
@@ -188,10 +207,10 @@ __global__ void k_dpm_scatter3(const float *x, const float *g, int64_t np, int n
 }
 
 /* xbar_i += sum_c dW(x_i,c)/dx * (s_lam lam_c + g_i . M_c)   (g, M may be null) */
-__global__ void k_dpm_grad(const float *x, int64_t np, int n, const float *lam, float s_lam, const float *g,
+__global__ void k_dpm_grad(const float *x, int64_t np, Lat L, int n, const float *lam, float s_lam, const float *g,
                            const float *M, float *xbar) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    const Cic c = cic_of(x + 3 * q, n);
+    const Cic c = cic_of(x + 3 * q, q, L, n);
     float gv[3] = {0.f, 0.f, 0.f};
     if (g) {
       gv[0] = g[3 * q];
@@ -250,6 +269,8 @@ __global__ void k_dpm_psibar(const float *xbar, const float *pbar, int64_t np, f
     CUBE_LAUNCH_CHECK();                                                                    \
   } while (0)
 
+static Lat lat(const cube_dpm *d) { return Lat{d->p.ng, d->p.nf / d->p.ng}; }
+
 static int zero(cube_dpm *d, float *p, int64_t n) {
   LAUNCH(k_dpm_zero, p, n);
   return CUBE_OK;
@@ -260,7 +281,7 @@ static int force_of(cube_dpm *d, const float *x) {
   const int nf = d->p.nf;
   int st = zero(d, d->rho, d->nf3);
   if (st) return st;
-  LAUNCH(k_dpm_deposit, x, d->np, nf, d->rho);
+  LAUNCH(k_dpm_deposit, x, d->np, lat(d), nf, d->rho);
   if ((st = cube_fft_r2c_3d(&d->c, d->plan_f, d->rho, d->spec_f))) return st;
   return cube_fft_green_c2r_3d(&d->c, d->plan_f, d->spec_f, d->kern, 0, d->work3_f, d->force, (float)nf, nullptr,
                                -1, (unsigned *)d->c.ured, d->c.slots);
@@ -271,10 +292,10 @@ static int step(cube_dpm *d, int s, int have_force) {
   int st;
   if (!have_force && (st = force_of(d, d->x))) return st;
   const double *c = d->steps + 3 * s;
-  LAUNCH(k_dpm_kick, d->x, d->pm, d->np, d->force, d->p.nf, (float)c[0]);
-  LAUNCH(k_dpm_drift, d->x, d->pm, d->np, d->p.nf, (float)c[1]);
+  LAUNCH(k_dpm_kick, d->x, d->pm, d->np, d->force, lat(d), d->p.nf, (float)c[0]);
+  LAUNCH(k_dpm_drift, d->x, d->pm, d->np, (float)c[1]);
   if ((st = force_of(d, d->x))) return st;
-  LAUNCH(k_dpm_kick, d->x, d->pm, d->np, d->force, d->p.nf, (float)c[2]);
+  LAUNCH(k_dpm_kick, d->x, d->pm, d->np, d->force, lat(d), d->p.nf, (float)c[2]);
   return CUBE_OK;
 }
 
@@ -317,13 +338,13 @@ static int vjp_force(cube_dpm *d, const float *x, float coef) {
   if ((st = force_of(d, x))) return st; /* M = F mesh at x */
   LAUNCH(k_dpm_scale_to, d->gvec, d->pbar, d->np * 3, coef);
   if ((st = zero(d, d->S, 3 * d->nf3))) return st;
-  LAUNCH(k_dpm_scatter3, x, d->gvec, d->np, nf, d->S);
+  LAUNCH(k_dpm_scatter3, x, d->gvec, d->np, lat(d), nf, d->S);
   for (int a = 0; a < 3; ++a)
     if ((st = cube_fft_forward_full(&d->c, d->plan_f, d->S + a * d->nf3, d->work3_f + a * d->sf))) return st;
   LAUNCH(k_dpm_combine, d->work3_f, d->sf, d->spec_f, (const float *)nullptr, nf);
   if ((st = cube_fft_inverse_full(&d->c, d->plan_f, d->spec_f, d->lam))) return st;
   const float s = 1.f / ((float)nf * (float)nf * (float)nf);
-  LAUNCH(k_dpm_grad, x, d->np, nf, d->lam, s, d->gvec, d->force, d->xbar);
+  LAUNCH(k_dpm_grad, x, d->np, lat(d), nf, d->lam, s, d->gvec, d->force, d->xbar);
   return CUBE_OK;
 }
 
@@ -427,7 +448,7 @@ extern "C" int cube_dpm_forward(cube_dpm *d, const float *noise, float *delta) {
                                 hipMemcpyDeviceToDevice, d->c.stream));
   }
   if ((st = zero(d, d->rho, d->nf3))) return st;
-  LAUNCH(k_dpm_deposit, d->x, d->np, d->p.nf, d->rho);
+  LAUNCH(k_dpm_deposit, d->x, d->np, lat(d), d->p.nf, d->rho);
   LAUNCH(k_dpm_delta, d->rho, d->nf3, (float)((double)d->nf3 / (double)d->np), delta);
   d->has_forward = 1;
   return CUBE_OK;
@@ -445,7 +466,7 @@ extern "C" int cube_dpm_adjoint(cube_dpm *d, const float *delta_bar, float *nois
   if ((st = positions_at(d, S, &xS))) return st;
   if ((st = zero(d, d->xbar, np3))) return st;
   if ((st = zero(d, d->pbar, np3))) return st;
-  LAUNCH(k_dpm_grad, xS, d->np, nf, delta_bar, (float)((double)d->nf3 / (double)d->np), (const float *)nullptr,
+  LAUNCH(k_dpm_grad, xS, d->np, lat(d), nf, delta_bar, (float)((double)d->nf3 / (double)d->np), (const float *)nullptr,
          (const float *)nullptr, d->xbar);
   for (int s = S - 1; s >= 0; --s) {
     const double *c = d->steps + 3 * s;
@@ -471,7 +492,7 @@ extern "C" int cube_dpm_state(cube_dpm *d, float *x, float *p) {
   CUBE_CHECK(hipSetDevice(d->p.device));
   const size_t b = d->np * 3 * sizeof(float);
   const float *xf = d->p.checkpoint ? d->xs + (int64_t)d->p.nsteps * d->np * 3 : d->x;
-  if (x) CUBE_CHECK(hipMemcpyAsync(x, xf, b, hipMemcpyDeviceToDevice, d->c.stream));
+  if (x) LAUNCH(k_dpm_positions, xf, d->np, lat(d), d->p.nf, x); /* held as displacements */
   if (p) CUBE_CHECK(hipMemcpyAsync(p, d->pm, b, hipMemcpyDeviceToDevice, d->c.stream));
   return CUBE_OK;
 }

@@ -8,7 +8,8 @@
  *   ICs      psi_d = (1/ng^3) IFFT[ i k_d/k^2 A(k) FFT(w) ]   on the ng^3
  *            particle lattice (w = white noise, A = amplitude per mode),
  *            x0 = q + cx * psi * (nf/ng),  p0 = cp * psi * (nf/ng)
- *            (q = lattice point, positions in mesh cells of the nf^3 mesh)
+ *            (q = lattice point, positions in mesh cells of the nf^3 mesh;
+ *            the context holds each particle as its FP32 displacement x - q)
  *   force    F(x) = CIC-gather of (1/nf^3) IFFT[ i k/k^2 FFT(CIC-deposit(x)) ]
  *   step s   p += kA_s F(x);  x += d_s p;  p += kB_s F(x)      (KDK)
  *   output   delta = CIC-deposit(x_final) / (np/nf^3) - 1 on the nf^3 mesh
@@ -60,8 +61,8 @@ int cube_dpm_forward(cube_dpm *d, const float *noise, float *delta);
 /* delta_bar: nf^3 (device) = dL/d(delta); noise_bar: ng^3 (device), written.
    Needs the forward of the same noise just before. */
 int cube_dpm_adjoint(cube_dpm *d, const float *delta_bar, float *noise_bar);
-/* final positions and momenta after the last forward, np*3 floats each
-   (device; either may be NULL) */
+/* final positions (wrapped into [0, nf)) and momenta after the last
+   forward, np*3 floats each (device; either may be NULL) */
 int cube_dpm_state(cube_dpm *d, float *x, float *p);
 /* wait for the context's stream */
 int cube_dpm_sync(cube_dpm *d);

@@ -1,7 +1,10 @@
 """Batched ensemble runner (include/cube_batch.h): the graph replay of the
 whole ensemble gives the per-box loop's losses and gradients, replay after
 replay with eager work in between, and each box equals a standalone box of
-the differentiable PM."""
+the differentiable PM; the full 48-box ensemble of uneven sizes (16x32^3,
+16x64^3, 8x96^3, 8x128^3 particles, 10 steps) gives finite losses, the
+loop's results from its graph, and a 32^3 box's gradient within 1e-3 of
+the FP64 twin's."""
 import numpy as np
 import pytest
 
@@ -67,3 +70,45 @@ def test_ensemble_box_equals_standalone_box():
     assert rel_l2(nb.cpu().numpy(), x["noise_bar"].cpu().numpy()) < 1e-4
     loss = float(torch.mean((d - x["target"]) ** 2).item())
     assert abs(loss / float(x["loss"].item()) - 1) < 1e-5
+
+
+def test_config5_full_ensemble():
+    import torch
+    from cafproject_amd import ics
+    from tests.dpm_ref import Ref
+    b, t = dpm.make_ensemble()  # 48 boxes, 10 steps
+    assert len(t) == 48
+    b.tune()
+    b.run(graph=False)
+    b.sync()
+    loop = _grab(t)
+    assert all(np.isfinite(l) for l, _, _ in loop)
+    b.run(graph=True)
+    b.sync()
+    graph = _grab(t)
+    # the deposits and their transposes add with FP32 atomics, so two runs of
+    # a box sum in different orders; over 10 steps the 96^3 and 128^3 boxes'
+    # gradients then differ by up to ~5e-4 run to run (measured), inside the
+    # 1e-3 asked of the gradient against FP64
+    worst = 0.0
+    for (l0, g0, d0), (l1, g1, d1) in zip(loop, graph):
+        assert np.isfinite(l1) and abs(l1 / l0 - 1) < 1e-5
+        assert rel_l2(d1, d0) < 1e-5
+        worst = max(worst, rel_l2(g1, g0))
+    print(f"graph vs loop: largest gradient difference {worst:.2e}", flush=True)
+    assert worst < 1e-3
+    # box 0 (32^3 particles, 64^3 mesh) against the FP64 twin
+    x = t[0]
+    ic, steps = dpm.schedule(0.1, 1.0, 10)
+    amp = dpm.amplitude(32, x["box"], ics.Cosmology(sigma8=x["sigma8"]))
+    ref = Ref(32, 64, amp.astype(np.float64), ic, steps)
+    w = x["noise"].cpu().numpy().astype(np.float64)
+    tgt = x["target"].cpu().numpy().astype(np.float64)
+    dr = ref.forward(w).ravel()
+    gr = ref.adjoint(2 * (dr - tgt) / dr.size)
+    _, g1, d1 = graph[0]
+    print(f"box 0 vs FP64 twin: delta {rel_l2(d1, dr):.2e}, gradient {rel_l2(g1, gr):.2e}", flush=True)
+    assert rel_l2(d1, dr) < 1e-3
+    assert rel_l2(g1, gr) < 1e-3
+    del b, t
+    torch.cuda.empty_cache()
