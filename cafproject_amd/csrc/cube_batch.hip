@@ -218,7 +218,10 @@ extern "C" int cube_batch_sync(cube_batch *b) {
 extern "C" int cube_batch_tune(cube_batch *b, int32_t *shapes) {
   if (!b) return CUBE_ERR_INVALID;
   CUBE_CHECK(hipSetDevice(b->device));
-  static const int cand[][2] = {{256, 256}, {1024, 256}, {2048, 256}, {4096, 256}, {1024, 512}, {512, 1024}};
+  /* grid x block of the model's grid-stride loops: from one workgroup per
+     CU to several per CU, at 2, 4, 8 and 16 waves per workgroup */
+  static const int cand[][2] = {{256, 256},  {512, 256},  {1024, 256}, {2048, 256}, {4096, 256}, {8192, 256},
+                                {1024, 128}, {2048, 128}, {4096, 128}, {512, 512},  {1024, 512}, {512, 1024}};
   const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
   int *done = (int *)calloc(b->nbox, sizeof(int));
   if (!done) return CUBE_ERR_ALLOC;

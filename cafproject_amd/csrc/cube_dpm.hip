@@ -130,7 +130,20 @@ __global__ void k_dpm_ic_particles(const float *psi, int ng, int nf, float cx, f
   }
 }
 
-__global__ void k_dpm_deposit(const float *x, int64_t np, Lat L, int n, float *rho) {
+/*
+ * The deposit and its transpose add particles' terms into mesh points from
+ * many threads at once.  They add 64-bit fixed-point integers, whose sums do
+ * not depend on the order of the atomics, and one pass converts to FP32
+ * afterwards: the meshes, and so the whole model, are the same bits run to
+ * run and in any launch shape or graph.  The deposit's terms are weights in
+ * [0, 1], kept with 32 fraction bits; the transpose's terms w*g are scaled
+ * by a power of two that puts max|g| in [2^39, 2^40) (found on the device
+ * before the scatter), so every sum of up to 2^23 terms stays in range.
+ * Each term is rounded once to the fixed-point grid, finer than FP32 at
+ * the terms' scale; the conversion rounds the exact sum once.
+ */
+#define FIX_DEPOSIT 0x1p32
+__global__ void k_dpm_deposit(const float *x, int64_t np, Lat L, int n, unsigned long long *acc) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     const Cic c = cic_of(x + 3 * q, q, L, n);
 #pragma unroll
@@ -139,9 +152,30 @@ __global__ void k_dpm_deposit(const float *x, int64_t np, Lat L, int n, float *r
       for (int y = 0; y < 2; ++y)
 #pragma unroll
         for (int xx = 0; xx < 2; ++xx)
-          atomicAdd(rho + ((int64_t)c.i[2][z] * n + c.i[1][y]) * n + c.i[0][xx],
-                    c.w[0][xx] * c.w[1][y] * c.w[2][z]);
+          atomicAdd(acc + ((int64_t)c.i[2][z] * n + c.i[1][y]) * n + c.i[0][xx],
+                    (unsigned long long)llrint((double)(c.w[0][xx] * c.w[1][y] * c.w[2][z]) * FIX_DEPOSIT));
   }
+}
+/* the transpose's scale: 2^(40-e) for max|g| = m 2^e, m in [0.5, 1) */
+__device__ static inline double fix_scale(const unsigned *gmx) {
+  int e = 0;
+  (void)frexp((double)__uint_as_float(*gmx), &e);
+  return ldexp(1.0, 40 - e);
+}
+/* out = acc / scale (scale: FIX_DEPOSIT, or the transpose's from gmx) */
+__global__ void k_dpm_from_fix(const unsigned long long *acc, int64_t n, const unsigned *gmx, float *out) {
+  const double inv = gmx ? 1.0 / fix_scale(gmx) : 1.0 / FIX_DEPOSIT;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    out[q] = (float)((double)(long long)acc[q] * inv);
+}
+/* max |g| over 3*np values into *gmx (non-negative floats order as their bits) */
+__global__ void k_dpm_absmax(const float *g, int64_t n, unsigned *gmx) {
+  float m = 0.f;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(g[q]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(gmx, __float_as_uint(m));
 }
 
 /* p += coef * sum_c W F_c (F interleaved, 3 per mesh point) */
@@ -186,8 +220,10 @@ __global__ void k_dpm_delta(const float *rho, int64_t n3, float inv_nbar, float 
 }
 
 /* S_d[c] += W(x_i,c) g_i,d  (SoA meshes) */
-__global__ void k_dpm_scatter3(const float *x, const float *g, int64_t np, Lat L, int n, float *S) {
+__global__ void k_dpm_scatter3(const float *x, const float *g, int64_t np, Lat L, int n, unsigned long long *S,
+                               const unsigned *gmx) {
   const int64_t n3 = (int64_t)n * n * n;
+  const double scale = fix_scale(gmx);
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     const Cic c = cic_of(x + 3 * q, q, L, n);
     const float gv[3] = {g[3 * q], g[3 * q + 1], g[3 * q + 2]};
@@ -199,9 +235,9 @@ __global__ void k_dpm_scatter3(const float *x, const float *g, int64_t np, Lat L
         for (int xx = 0; xx < 2; ++xx) {
           const float w = c.w[0][xx] * c.w[1][y] * c.w[2][z];
           const int64_t m = ((int64_t)c.i[2][z] * n + c.i[1][y]) * n + c.i[0][xx];
-          atomicAdd(S + m, w * gv[0]);
-          atomicAdd(S + n3 + m, w * gv[1]);
-          atomicAdd(S + 2 * n3 + m, w * gv[2]);
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            atomicAdd(S + a * n3 + m, (unsigned long long)llrint((double)(w * gv[a]) * scale));
         }
   }
 }
@@ -279,9 +315,10 @@ static int zero(cube_dpm *d, float *p, int64_t n) {
 /* force mesh of positions x into d->force (interleaved 3 per point) */
 static int force_of(cube_dpm *d, const float *x) {
   const int nf = d->p.nf;
-  int st = zero(d, d->rho, d->nf3);
+  int st = zero(d, reinterpret_cast<float *>(d->acc), 2 * d->nf3);
   if (st) return st;
-  LAUNCH(k_dpm_deposit, x, d->np, lat(d), nf, d->rho);
+  LAUNCH(k_dpm_deposit, x, d->np, lat(d), nf, d->acc);
+  LAUNCH(k_dpm_from_fix, d->acc, d->nf3, (const unsigned *)nullptr, d->rho);
   if ((st = cube_fft_r2c_3d(&d->c, d->plan_f, d->rho, d->spec_f))) return st;
   return cube_fft_green_c2r_3d(&d->c, d->plan_f, d->spec_f, d->kern, 0, d->work3_f, d->force, (float)nf, nullptr,
                                -1, (unsigned *)d->c.ured, d->c.slots);
@@ -337,8 +374,11 @@ static int vjp_force(cube_dpm *d, const float *x, float coef) {
   int st;
   if ((st = force_of(d, x))) return st; /* M = F mesh at x */
   LAUNCH(k_dpm_scale_to, d->gvec, d->pbar, d->np * 3, coef);
-  if ((st = zero(d, d->S, 3 * d->nf3))) return st;
-  LAUNCH(k_dpm_scatter3, x, d->gvec, d->np, lat(d), nf, d->S);
+  if ((st = zero(d, reinterpret_cast<float *>(d->acc), 6 * d->nf3))) return st;
+  if ((st = zero(d, reinterpret_cast<float *>(d->gmx), 1))) return st;
+  LAUNCH(k_dpm_absmax, d->gvec, 3 * d->np, d->gmx);
+  LAUNCH(k_dpm_scatter3, x, d->gvec, d->np, lat(d), nf, d->acc, d->gmx);
+  LAUNCH(k_dpm_from_fix, d->acc, 3 * d->nf3, d->gmx, d->S);
   for (int a = 0; a < 3; ++a)
     if ((st = cube_fft_forward_full(&d->c, d->plan_f, d->S + a * d->nf3, d->work3_f + a * d->sf))) return st;
   LAUNCH(k_dpm_combine, d->work3_f, d->sf, d->spec_f, (const float *)nullptr, nf);
@@ -355,7 +395,8 @@ static int dalloc_f(float **p, int64_t n) {
 extern "C" void cube_dpm_destroy(cube_dpm *d) {
   if (!d) return;
   void *ptrs[] = {d->amp, d->kern, d->xs, d->x, d->pm, d->p0, d->rho, d->force, d->S, d->lam, d->spec_f,
-                  d->work3_f, d->spec_g, d->work3_g, d->psi, d->xbar, d->pbar, d->gvec, d->c.ured, d->c.slots};
+                  d->work3_f, d->spec_g, d->work3_g, d->psi, d->xbar, d->pbar, d->gvec, d->c.ured, d->c.slots,
+                  d->acc, d->gmx};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   cube_fft_destroy(d->plan_g);
@@ -397,6 +438,8 @@ extern "C" int cube_dpm_create(const cube_dpm_params *p, cube_dpm **out) {
   if (!st) st |= dalloc_f(&d->rho, d->nf3);
   if (!st) st |= dalloc_f(&d->force, 3 * d->nf3);
   if (!st) st |= dalloc_f(&d->S, 3 * d->nf3);
+  if (!st) st |= dalloc_f(reinterpret_cast<float **>(&d->acc), 6 * d->nf3);
+  if (!st) st |= dalloc_f(reinterpret_cast<float **>(&d->gmx), 1);
   if (!st) st |= dalloc_f(&d->lam, d->nf3);
   if (!st) st |= dalloc_f((float **)&d->spec_f, 2 * d->sf);
   if (!st) st |= dalloc_f((float **)&d->work3_f, 6 * d->sf);
@@ -447,8 +490,9 @@ extern "C" int cube_dpm_forward(cube_dpm *d, const float *noise, float *delta) {
       CUBE_CHECK(hipMemcpyAsync(d->xs + (int64_t)(s + 1) * d->np * 3, d->x, d->np * 3 * sizeof(float),
                                 hipMemcpyDeviceToDevice, d->c.stream));
   }
-  if ((st = zero(d, d->rho, d->nf3))) return st;
-  LAUNCH(k_dpm_deposit, d->x, d->np, lat(d), d->p.nf, d->rho);
+  if ((st = zero(d, reinterpret_cast<float *>(d->acc), 2 * d->nf3))) return st;
+  LAUNCH(k_dpm_deposit, d->x, d->np, lat(d), d->p.nf, d->acc);
+  LAUNCH(k_dpm_from_fix, d->acc, d->nf3, (const unsigned *)nullptr, d->rho);
   LAUNCH(k_dpm_delta, d->rho, d->nf3, (float)((double)d->nf3 / (double)d->np), delta);
   d->has_forward = 1;
   return CUBE_OK;

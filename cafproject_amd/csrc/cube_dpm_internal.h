@@ -19,6 +19,8 @@ struct cube_dpm {
   float *xs;            /* checkpoints: (nsteps+1 or 1) * np*3 */
   float *x, *pm, *p0;   /* current positions, momenta; ICs' momenta */
   float *rho, *force, *S, *lam;
+  unsigned long long *acc; /* 3*nf^3 fixed-point sums of the deposit / its transpose */
+  unsigned *gmx;           /* max |g| of the transpose's cotangents (float bits) */
   float2 *spec_f, *work3_f;
   float2 *spec_g, *work3_g;
   float *psi;           /* 3*ng^3 */

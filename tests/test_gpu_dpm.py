@@ -104,8 +104,8 @@ def test_recompute_matches_checkpoints():
     box2, *_ = _setup(checkpoint=False)
     d2 = _fwd(box2, w, torch)
     g2 = _adj(box2, 2 * (d2 - target) / d2.size, torch)
-    assert rel_l2(d2, d1) < 1e-5
-    assert rel_l2(g2, g1) < 1e-4
+    assert np.array_equal(d2, d1)  # the same steps, order-free sums: the same bits
+    assert np.array_equal(g2, g1)
 
 
 def test_autograd_function_gives_the_adjoint():
