@@ -354,6 +354,7 @@ __global__ void k_place_ranked(const int32_t *__restrict__ dest, const int32_t *
 
 /* ---------------- per-cell sort ---------------- */
 #define SMALL_SORT 48
+#define MED_SORT 1024
 #define BIG_SORT 16384
 #define FB_CELLS 64      /* cells per finalize workgroup (the context sizes fbflag/vpart for it) */
 /* slots staged in LDS per finalize pass: 640 (10 per cell against the 8 of
@@ -381,9 +382,14 @@ __global__ void k_sort_small(const int32_t *__restrict__ cnt, const int64_t *__r
   if (skip && skip[cidx / FB_CELLS]) return; /* sorted by its finalize workgroup */
   const int n = cnt[cidx];
   if (n <= 1) return;
-  if (n > SMALL_SORT) {
+  if (n > MED_SORT) { /* big cells from the front of the list */
     const unsigned long long q = atomicAdd(counters + 3, 1ull);
     biglist[q] = (int32_t)cidx;
+    return;
+  }
+  if (n > SMALL_SORT) { /* medium cells from the back */
+    const unsigned long long q = atomicAdd(counters + 4, 1ull);
+    biglist[ncells - 1 - (int64_t)q] = (int32_t)cidx;
     return;
   }
   uint64_t v[SMALL_SORT];
@@ -431,6 +437,48 @@ __device__ static int lower_bound_u64(const uint64_t *r, int n, uint64_t x) {
     if (r[mid] < x) lo = mid + 1; else hi = mid;
   }
   return lo;
+}
+
+/* One wave per medium cell (SMALL_SORT < n <= MED_SORT), four per
+   workgroup: a bitonic sort of the cell's keys (padded to a power of two,
+   at least 64) in the wave's 8 KB of LDS, the stages ordered by the wave
+   itself -- no workgroup barriers, and 32 KB per workgroup so that five
+   share a CU where a big cell's workgroup takes 128 KB. */
+__global__ __launch_bounds__(256) void k_sort_med(const int32_t *__restrict__ cnt, const int64_t *__restrict__ start,
+                                                  const int32_t *__restrict__ list, int64_t nlist,
+                                                  uint64_t *__restrict__ a) {
+  __shared__ uint64_t sm[4 * MED_SORT];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 4 + wv;
+  if (i >= nlist) return; /* the whole wave */
+  uint64_t *sk = sm + wv * MED_SORT;
+  const int32_t cidx = list[i];
+  const int n = cnt[cidx];
+  uint64_t *s = a + start[cidx];
+  int m = 64;
+  while (m < n) m <<= 1;
+  for (int q = lane; q < m; q += 64) sk[q] = q < n ? s[q] : ~0ull;
+  for (int size = 2; size <= m; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int q = lane; q < m; q += 64) { /* each pair (q, q^stride) has one owner */
+        const int o = q ^ stride;
+        if (o > q) {
+          const bool up = (q & size) == 0;
+          const uint64_t x = sk[q], y = sk[o];
+          if ((x > y) == up) {
+            sk[q] = y;
+            sk[o] = x;
+          }
+        }
+      }
+    }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int q = lane; q < n; q += 64) s[q] = sk[q];
 }
 
 /* One workgroup per big cell.  Up to BIG_SORT keys: a bitonic sort in LDS.
@@ -759,12 +807,13 @@ __global__ void k_place_keys(const int32_t *dest, const uint64_t *key, int64_t n
     out[start[d] + s] = key[p];
   }
 }
-/* sort every cell's key segment ascending (small ones in registers, big
-   ones with a workgroup bitonic sort, huge ones by merging sorted runs);
+/* sort every cell's key segment ascending (small ones in registers, medium
+   ones with a wave's bitonic sort, big ones with a workgroup's, huge ones by
+   merging sorted runs);
    c->key, dead once the keys are placed, is the merge buffer */
 extern "C" int cube_sort_small_segments(cube_ctx *c, const int32_t *cnt, const int64_t *start, uint64_t *keys) {
   unsigned long long *ctr = (unsigned long long *)c->ured;
-  int st = fill_i64(c, (int64_t *)ctr + 3, 1, 0);
+  int st = fill_i64(c, (int64_t *)ctr + 3, 2, 0);
   if (st) return st;
   const unsigned nbc = (unsigned)((c->ncells + 255) / 256);
   int32_t *biglist = (int32_t *)c->dest;
@@ -773,6 +822,11 @@ extern "C" int cube_sort_small_segments(cube_ctx *c, const int32_t *cnt, const i
   CUBE_LAUNCH_CHECK();
   unsigned long long hc[8];
   CUBE_CHECK(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
+  if (hc[4]) {
+    hipLaunchKernelGGL(k_sort_med, dim3((unsigned)((hc[4] + 3) / 4)), dim3(256), 0, c->stream, cnt, start,
+                       biglist + c->ncells - (int64_t)hc[4], (int64_t)hc[4], keys);
+    CUBE_LAUNCH_CHECK();
+  }
   if (hc[3]) {
     hipLaunchKernelGGL(k_sort_big, dim3((unsigned)hc[3]), dim3(1024), BIG_SORT * sizeof(uint64_t), c->stream, cnt,
                        start, biglist, keys, c->key);
@@ -821,6 +875,11 @@ template <int IX, int IV> static int drift_impl(cube_ctx *c, DriftP P, double ve
                      biglist, ctr, c->fbflag);
   CUBE_LAUNCH_CHECK();
   CUBE_CHECK(hipMemcpy(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  if (hc[4]) {
+    hipLaunchKernelGGL(k_sort_med, dim3((unsigned)((hc[4] + 3) / 4)), dim3(256), 0, c->stream, c->rhoc2,
+                       c->cstart2, biglist + c->ncells - (int64_t)hc[4], (int64_t)hc[4], c->sortbuf);
+    CUBE_LAUNCH_CHECK();
+  }
   if (hc[3]) {
     hipLaunchKernelGGL(k_sort_big, dim3((unsigned)hc[3]), dim3(1024), BIG_SORT * sizeof(uint64_t), c->stream,
                        c->rhoc2, c->cstart2, biglist, c->sortbuf, c->key); /* key is dead after placement */
