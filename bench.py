@@ -321,17 +321,19 @@ def pair_count(st, nc, nnt):
     return pairs
 
 
-def alone(st, nc, nnt, izipx, a_mid, dt, sigma, mass, reps=3):
-    """PP and both deposits on a copy of the state in a second context
-    without Green's functions (so nothing runs beside PP): fastest of reps."""
+def alone(st, nc, nnt, izipx, a_mid, dt, sigma, mass, drift_args, reps=3):
+    """PP, both deposits and the drift on a copy of the state in a second
+    context without Green's functions (so nothing runs beside PP): fastest
+    of reps."""
     from cafproject_amd.cube import Cube
     g2 = Cube(nc, nnt, izipx=izipx, izipv=2, has_pid=False, pp_on=True, pp_range=1, rsoft=0.3)
-    best = dict(pp=1e30, fine_deposit=1e30, coarse_deposit=1e30)
+    best = dict(pp=1e30, fine_deposit=1e30, coarse_deposit=1e30, drift=1e30)
     for _ in range(reps):
         g2.set_state(st["xp"], st["vp"], None, st["rhoc"], st["vfield"])
         g2.sync()
         for k, fn in (("pp", lambda: g2.ext_pp_force(a_mid, dt, sigma, mass)),
-                      ("fine_deposit", lambda: g2.fine_deposit(mass)), ("coarse_deposit", lambda: g2.coarse_deposit(mass))):
+                      ("fine_deposit", lambda: g2.fine_deposit(mass)), ("coarse_deposit", lambda: g2.coarse_deposit(mass)),
+                      ("drift", lambda: g2.update_x(*drift_args))):
             t = time.perf_counter()
             fn()
             g2.sync()
@@ -345,8 +347,8 @@ def to_z0(nc, nnt, box, izipx, pp_on, max_steps=3000):
     reference's step criterion (W/main.f90:88-145): every step's wall time
     and stage times (PP, deposits, drift, kicks) binned by redshift, the
     particle count after every step, P(k) of the final state, and at z = 1
-    and z = 0 the state's PP pair count with PP and the deposits timed alone
-    (outside the run's clock)."""
+    and z = 0 the state's PP pair count with PP, the deposits and the drift
+    timed alone (outside the run's clock)."""
     from cafproject_amd.cube import Cube
     g = Cube(nc, nnt, izipx=izipx, izipv=2, has_pid=False, pp_on=pp_on, pp_range=1, rsoft=0.3)
     g.init_kernels()
@@ -380,7 +382,8 @@ def to_z0(nc, nnt, box, izipx, pp_on, max_steps=3000):
             tp = time.perf_counter()
             st_h = g.get_state()
             pairs = pair_count(st_h, nc, nnt)
-            al = alone(st_h, nc, nnt, izipx, float(step.clock.a), float(step.clock.dt), step.sigma, mass)
+            al = alone(st_h, nc, nnt, izipx, float(step.clock.a), float(step.clock.dt), step.sigma, mass,
+                       (float(step.clock.dt_old), float(step.clock.dt), step.vz_max, step.sigma))
             del st_h
             probes.append(dict(z=round(1 / a1 - 1, 4), pairs=pairs, pairs_per_particle=round(pairs / n, 2),
                                alone_ms=al, pp_pairs_per_s=pairs / (al["pp"] * 1e-3)))
