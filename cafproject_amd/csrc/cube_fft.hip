@@ -18,9 +18,10 @@
  *   k_c2r_x3  : rows along x for the three spectra: merge -> complex N/2
  *               inverse FFT -> real, /N/N/N, write F as 3 floats per cell,
  *               and max |F|^2 (W/pm.f90:126,315)
- * Strided lines of power-of-two meshes 64..1024 (k_c2c_r, k_green_r) are
- * register-resident four-step FFTs over 16-column batches; other sizes and
- * the row passes use a Stockham autosort FFT in LDS (radix 8/4/2/3/5/7).
+ * Strided lines of power-of-two meshes 64..2048 (k_c2c_r, k_green_r) are
+ * register-resident four-step FFTs over 16-column batches (8 at 2048);
+ * other sizes and the row passes use a Stockham autosort FFT in LDS
+ * (radix 8/4/2/3/5/7).
  */
 #include "cube_internal.h"
 #include <string.h>
@@ -239,10 +240,10 @@ template <int N> struct Cfg {
 };
 
 /* ---------------- register-resident line FFTs ----------------
- * For power-of-two N = R1*R2 (64..1024) a strided line FFT keeps the line
+ * For power-of-two N = R1*R2 (64..2048) a strided line FFT keeps the line
  * in registers: a workgroup takes CB=16 adjacent columns (128-byte global
- * segments) and R1 threads per column, thread t holding x[t + R1*m],
- * m < R2.  Four-step: R2-point FFT in registers, twiddle by W_N^{t*q},
+ * segments; 8 columns at N=2048) and R1 threads per column, thread t
+ * holding x[t + R1*m], m < R2.  Four-step: R2-point FFT in registers, twiddle by W_N^{t*q},
  * one transpose through LDS, R1-point FFTs in registers; the result comes
  * back in the same distribution (thread t holds X[t + R1*m]).  One LDS
  * round trip per line instead of one per radix pass, and 16-column
@@ -330,18 +331,34 @@ __device__ __forceinline__ int opaque_zero() {
   asm volatile("s_mov_b32 %0, 0" : "=s"(z));
   return z;
 }
+/* the same zero, formed only once `after` has been computed: loads offset
+   by it cannot be hoisted above the code that produces `after` */
+__device__ __forceinline__ int opaque_zero_after(float after) {
+  int z = 0;
+  asm volatile("s_mov_b32 %0, 0 ; %1" : "=s"(z) : "v"(after));
+  return z;
+}
 
 template <int N> struct RC {
-  static constexpr bool ok = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024;
+  static constexpr bool ok = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024 || N == 2048;
   static constexpr int R1 = N <= 128 ? 8 : (N <= 512 ? 16 : 32);
   static constexpr int R2 = N / R1;
-  static constexpr int CB = 16;
+  /* 16 columns (128-byte pieces) per workgroup; at N=2048 8 columns, so
+     that the transpose buffer (67.8 KB) lets two workgroups share a CU */
+  static constexpr int CB = N >= 2048 ? 8 : 16;
   static constexpr int NT = CB * R1;
-  /* LDS transpose buffer S[c][q][t]: q stride R1+1, c stride = 4 (mod 64)
-     so both the row writes and the column reads of a wave hit distinct banks */
+  /* LDS transpose buffer S[c][q][t]: q stride R1+1, c stride = 64/CB
+     (mod 64) so both the row writes and the column reads of a wave hit
+     distinct banks */
   static constexpr int SQ = R1 + 1;
   static constexpr int SC0 = R2 * SQ;
-  static constexpr int SC = SC0 + ((4 - SC0) % 64 + 64) % 64;
+  static constexpr int SC = SC0 + ((64 / CB - SC0) % 64 + 64) % 64;
+  /* the twiddle table W_N^e, e < N, staged in LDS (up to 8 KB); at N=2048
+     it is read from global memory (cached) to leave LDS to the transpose */
+  static constexpr bool TWL = N <= 1024;
+  /* the Green z-pass forms one force component per launch (the forward
+     transform kept in the registers it is done in: 64 values per thread) */
+  static constexpr bool ONE = N >= 2048;
 };
 
 /* line FFT of the CB columns of a workgroup; on entry v[m] = x[t + R1 m] of
@@ -407,16 +424,18 @@ __device__ __forceinline__ void blk_fft(float2 *v, int c, int t, float *S, const
 
 /* in-place strided line FFTs, register version: element (col, n) of line
    set (blockIdx.y, blockIdx.z) at a + z*cstride + y*ostride + n*stride + col.
-   Requires R1*stride*8 < 2^32 (true for every mesh up to 1024^3). */
+   Requires R1*stride*8 < 2^32 (true for every mesh up to 2048^3). */
 template <int N, int DIR>
 __global__ __launch_bounds__(RC<N>::NT) void k_c2c_r(float2 *__restrict__ a, const float2 *__restrict__ tw,
                                                    int ncol, int64_t stride, int64_t ostride, int64_t cstride) {
   using G = RC<N>;
   constexpr int R1 = G::R1, R2 = G::R2;
   __shared__ float S[G::CB * G::SC];
-  __shared__ float2 twl[N];
+  __shared__ float2 twls[G::TWL ? N : 1];
   const int tid = threadIdx.x, c = tid % G::CB, t = tid / G::CB;
-  for (int e = tid; e < N; e += G::NT) twl[e] = tw[e];
+  if constexpr (G::TWL)
+    for (int e = tid; e < N; e += G::NT) twls[e] = tw[e];
+  const float2 *twl = G::TWL ? twls : tw;
   const int col = blockIdx.x * G::CB + c;
   const bool ok = col < ncol;
   /* columns past the end compute on a copy of the last one and store nothing */
@@ -449,12 +468,18 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
   /* the forward z spectrum is needed by all three inverse transforms; at
      N=1024 (512 threads, 256 registers each) its upper half waits in LDS
      instead of registers, which would otherwise spill */
-  constexpr int KL = N >= 1024 ? R2 / 2 : 0;
+  static_assert(H % R1 == 0, "the octant offsets below take the side of H from m alone");
+  constexpr int KL = N == 1024 ? R2 / 2 : 0;
   __shared__ float S[G::CB * G::SC];
-  __shared__ float2 twl[N];
+  __shared__ float2 twls[G::TWL ? N : 1];
   __shared__ float2 keep[KL > 0 ? KL * G::NT : 1];
   const int tid = threadIdx.x, c = tid % G::CB, t = tid / G::CB;
-  for (int e = tid; e < N; e += G::NT) twl[e] = tw[e];
+  if constexpr (G::TWL)
+    for (int e = tid; e < N; e += G::NT) twls[e] = tw[e];
+  const float2 *twl = G::TWL ? twls : tw;
+  /* one component per launch: the loop below runs once, so the forward
+     spectrum v dies as w is formed from it and the two share registers */
+  if constexpr (G::ONE) d1 = d0 + 1;
   const int yp = blockIdx.y;
   const int ny = (yp == 0 || yp == H) ? 1 : 2;
 #pragma unroll 1
@@ -479,7 +504,9 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
       const float sy = y <= H ? 1.f : -1.f;
 #pragma unroll 1
       for (int d = d0; d < d1; ++d) {
-        const int z0 = opaque_zero();
+        /* at N=2048 the kernel loads wait for the forward transform, whose
+           peak register use they would otherwise add to */
+        const int z0 = G::ONE ? opaque_zero_after(v[R2 - 1].x) : opaque_zero();
         float2 w[R2];
 #pragma unroll
         for (int m = 0; m < R2; ++m) {
@@ -488,14 +515,22 @@ __global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict_
           float K;
           if constexpr (OCT) {
             constexpr int KB = (int)cube_kernf_blocks(H1);
-            const int kz = n <= H ? n : N - n;
-            const float *kb = uni(kern + ((int64_t)d * H1 + ky) * KB * H1 * 16, z0);
-            K = *boff(kb, (uint32_t)((((kx >> 4) * H1 + kz) * 16 + (kx & 15)) * 4));
+            /* kz = n for n <= H, N - n above; with H a multiple of R1 the
+               side is fixed by m alone (n = H only at m = H/R1, t = 0, where
+               both forms agree).  The part of the offset that depends on m
+               goes into the uniform base, so the 64 loads share two
+               per-lane offsets instead of holding 64 */
+            const float *kb = kern + ((int64_t)d * H1 + ky) * KB * H1 * 16;
+            const uint32_t kxo = (uint32_t)(((kx >> 4) * H1 * 16 + (kx & 15)) * 4);
+            if (R1 * m + R1 - 1 <= H)
+              K = *boff(uni(kb + 16 * R1 * m, z0), kxo + (uint32_t)(64 * t));
+            else
+              K = *boff(uni(kb + 16 * (N - R1 + 1 - R1 * m), z0), kxo + (uint32_t)(64 * (R1 - 1 - t)));
             if (d == 1) K *= sy;
             if (d == 2 && n > H) K = -K;
-          } else {
-            const float *kb = uni(kern + ((int64_t)d * N * N + y) * NH, z0);
-            K = *boff(kb, (uint32_t)((n * N * NH + kx) * 4));
+          } else { /* element ((d*N + n)*N + y)*NH + kx, n = t + R1*m: m in the base */
+            const float *kb = uni(kern + (((int64_t)d * N + R1 * m) * N + y) * NH, z0);
+            K = *boff(kb, (uint32_t)(((int64_t)t * N * NH + kx) * 4));
           }
           /* rho_f(::2)=-crho(2::2)*K, rho_f(2::2)=crho(::2)*K  (W/pm.f90:119-120) */
           w[m] = make_float2((-vm.y) * K, vm.x * K);
@@ -963,12 +998,16 @@ template <int N> struct Launch {
     if constexpr (RC<N>::ok) {
       /* pairs of column blocks x the line pairs (y, N-y), y = 0..N/2 */
       dim3 g(((NH + RC<N>::CB - 1) / RC<N>::CB + 1) / 2, N / 2 + 1);
-      if (oct)
-        hipLaunchKernelGGL((k_green_r<N, true>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw, d0,
-                           d1);
-      else
-        hipLaunchKernelGGL((k_green_r<N, false>), g, dim3(RC<N>::NT), 0, c->stream, spec, work3, kern, p->tw, d0,
-                           d1);
+      const int64_t plane = (int64_t)cube_spec_pitch(N) * N * N;
+      for (int e = d0; e < d1; e += RC<N>::ONE ? 1 : d1 - d0) {
+        const int e1 = RC<N>::ONE ? e + 1 : d1;
+        float2 *o = work3 + (e - d0) * plane;
+        if (oct)
+          hipLaunchKernelGGL((k_green_r<N, true>), g, dim3(RC<N>::NT), 0, c->stream, spec, o, kern, p->tw, e, e1);
+        else
+          hipLaunchKernelGGL((k_green_r<N, false>), g, dim3(RC<N>::NT), 0, c->stream, spec, o, kern, p->tw, e, e1);
+        CUBE_LAUNCH_CHECK();
+      }
     } else {
       dim3 g((NH + Cfg<N>::BS - 1) / Cfg<N>::BS, N);
       hipLaunchKernelGGL(k_green_z<N>, g, dim3(FFT_T), 0, c->stream, spec, work3, kern, oct, p->tw, d0, d1);

@@ -8,10 +8,10 @@ closed forms.
   the 24-cell tile buffer, so the two agree to FP32 rounding;
 * coarse force of a random density against the oracle's pencil solve
   (W/pm.f90:275-315);
-* at the 1024^3 fine mesh (nc=256), where the oracle is too slow for a
-  test: a density of plane waves cos(2 pi k.r/N) has the exact force
-  -K_d(k) sin(2 pi k.r/N) per wave, with K_d(k) read back from the
-  library's own kern_f (octant storage, odd in k_d).
+* at the 1024^3 and 2048^3 fine meshes (nc=256, 512), where the oracle is
+  too slow for a test: a density of plane waves cos(2 pi k.r/N) has the
+  exact force -K_d(k) sin(2 pi k.r/N) per wave, with K_d(k) read back from
+  the library's own kern_f (octant storage, odd in k_d).
 """
 import numpy as np
 import pytest
@@ -111,6 +111,57 @@ def test_plane_waves_1024():
             s = np.sin(2 * np.pi * (kx * x[None, :] + ky * x[:, None] + kz * z) / N + ph)
             want += -amp * s[..., None] * _kd(kern, k, N)[None, None, :]
         err = rel_l2(f[z], want)
+        assert err < 1e-5, (z, err)
+
+
+def test_plane_waves_2048():
+    """The same closed form at the 2048^3 fine mesh (nc=512, the compact
+    layout: one force spectrum at a time), whose strided passes run with 8
+    columns per workgroup and 32 x 64 register lines.  The density is formed
+    on the GPU by torch, 32 z planes at a time, and crosses to the library
+    through host memory (32 GB); the force comes back whole (96 GB) and four
+    z planes are checked."""
+    import gc
+    import torch
+    from cafproject_amd.cube import Cube
+    nc, N = 512, 2048
+    waves = [((3, 5, 7), 1.0, 0.3), ((1, N - 2, 4), 0.5, 1.1), ((17, 9, N - 30), 0.25, 2.0)]
+    rho = np.empty((N, N, N), np.float32)
+    dev = torch.device("cuda:0")
+    xi = torch.arange(N, device=dev, dtype=torch.int64)
+    ch = 32
+    for z0 in range(0, N, ch):
+        zi = torch.arange(z0, z0 + ch, device=dev, dtype=torch.int64)
+        acc = torch.zeros((ch, N, N), device=dev, dtype=torch.float64)
+        for (kx, ky, kz), amp, ph in waves:
+            m = (kz * zi[:, None, None] + ky * xi[None, :, None] + kx * xi[None, None, :]) % N
+            acc += amp * torch.cos(m.to(torch.float64) * (2 * np.pi / N) + ph)
+        rho[z0:z0 + ch] = acc.to(torch.float32).cpu().numpy()
+        del acc, m
+    del xi, zi
+    torch.cuda.empty_cache()
+    print("plane waves 2048: density formed", flush=True)
+    g = Cube(nc, 4, np_nc=1)
+    g.init_kernels()
+    g.set_field("rho_f", rho)
+    del rho
+    gc.collect()
+    g.fine_force()
+    print("plane waves 2048: force solved", flush=True)
+    f = g.get_field("force_f")
+    kern = g.get_field("kern_f")
+    kd = [_kd(kern, k, N) for k, _, _ in waves]
+    del kern, g
+    gc.collect()
+    x = np.arange(N, dtype=np.int64)
+    for z in (0, 1, 777, 2047):
+        want = np.zeros((N, N, 3))
+        for ((kx, ky, kz), amp, ph), K in zip(waves, kd):
+            m = (kx * x[None, :] + ky * x[:, None] + kz * z) % N
+            s = np.sin(2 * np.pi * m / N + ph)
+            want += -amp * s[..., None] * K[None, None, :]
+        err = rel_l2(f[z], want)
+        print(f"plane waves 2048: z={z} rel L2 {err:.2e}", flush=True)
         assert err < 1e-5, (z, err)
 
 
