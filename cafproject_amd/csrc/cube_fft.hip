@@ -19,9 +19,8 @@
  *               inverse FFT -> real, /N/N/N, write F as 3 floats per cell,
  *               and max |F|^2 (W/pm.f90:126,315)
  * Strided lines of power-of-two meshes 64..2048 (k_c2c_r, k_green_r) are
- * register-resident four-step FFTs over 16-column batches (8 at 2048);
- * other sizes and the row passes use a Stockham autosort FFT in LDS
- * (radix 8/4/2/3/5/7).
+ * register-resident four-step FFTs over 16-column batches; other sizes and
+ * the row passes use a Stockham autosort FFT in LDS (radix 8/4/2/3/5/7).
  */
 #include "cube_internal.h"
 #include <string.h>
@@ -242,8 +241,8 @@ template <int N> struct Cfg {
 /* ---------------- register-resident line FFTs ----------------
  * For power-of-two N = R1*R2 (64..2048) a strided line FFT keeps the line
  * in registers: a workgroup takes CB=16 adjacent columns (128-byte global
- * segments; 8 columns at N=2048) and R1 threads per column, thread t
- * holding x[t + R1*m], m < R2.  Four-step: R2-point FFT in registers, twiddle by W_N^{t*q},
+ * segments) and R1 threads per column, thread t holding x[t + R1*m],
+ * m < R2.  Four-step: R2-point FFT in registers, twiddle by W_N^{t*q},
  * one transpose through LDS, R1-point FFTs in registers; the result comes
  * back in the same distribution (thread t holds X[t + R1*m]).  One LDS
  * round trip per line instead of one per radix pass, and 16-column
@@ -343,9 +342,9 @@ template <int N> struct RC {
   static constexpr bool ok = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024 || N == 2048;
   static constexpr int R1 = N <= 128 ? 8 : (N <= 512 ? 16 : 32);
   static constexpr int R2 = N / R1;
-  /* 16 columns (128-byte pieces) per workgroup; at N=2048 8 columns, so
-     that the transpose buffer (67.8 KB) lets two workgroups share a CU */
-  static constexpr int CB = N >= 2048 ? 8 : 16;
+  /* 16 columns (128-byte pieces) per workgroup; at N=2048 the transpose
+     buffer is then 135 KB, one 512-thread workgroup per CU */
+  static constexpr int CB = 16;
   static constexpr int NT = CB * R1;
   /* LDS transpose buffer S[c][q][t]: q stride R1+1, c stride = 64/CB
      (mod 64) so both the row writes and the column reads of a wave hit
@@ -997,15 +996,16 @@ template <int N> struct Launch {
     constexpr int NH = N / 2 + 1;
     if constexpr (RC<N>::ok) {
       /* pairs of column blocks x the line pairs (y, N-y), y = 0..N/2 */
-      dim3 g(((NH + RC<N>::CB - 1) / RC<N>::CB + 1) / 2, N / 2 + 1);
+      using G = RC<N>;
+      dim3 g(((NH + G::CB - 1) / G::CB + 1) / 2, N / 2 + 1);
       const int64_t plane = (int64_t)cube_spec_pitch(N) * N * N;
-      for (int e = d0; e < d1; e += RC<N>::ONE ? 1 : d1 - d0) {
-        const int e1 = RC<N>::ONE ? e + 1 : d1;
+      for (int e = d0; e < d1; e += G::ONE ? 1 : d1 - d0) {
+        const int e1 = G::ONE ? e + 1 : d1;
         float2 *o = work3 + (e - d0) * plane;
         if (oct)
-          hipLaunchKernelGGL((k_green_r<N, true>), g, dim3(RC<N>::NT), 0, c->stream, spec, o, kern, p->tw, e, e1);
+          hipLaunchKernelGGL((k_green_r<N, true>), g, dim3(G::NT), 0, c->stream, spec, o, kern, p->tw, e, e1);
         else
-          hipLaunchKernelGGL((k_green_r<N, false>), g, dim3(RC<N>::NT), 0, c->stream, spec, o, kern, p->tw, e, e1);
+          hipLaunchKernelGGL((k_green_r<N, false>), g, dim3(G::NT), 0, c->stream, spec, o, kern, p->tw, e, e1);
         CUBE_LAUNCH_CHECK();
       }
     } else {
