@@ -458,7 +458,7 @@ __global__ __launch_bounds__(RC<N>::NT) void k_c2c_r(float2 *__restrict__ a, con
    full one (coarse mesh).  A workgroup does the lines y and N-y, which read
    the same kernel block, for two adjacent column blocks. */
 template <int N, bool OCT>
-__global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *__restrict__ spec, float2 *__restrict__ outd,
+__global__ __launch_bounds__(RC<N>::NT) void k_green_r(const float2 *spec, float2 *outd,
                                                      const float *__restrict__ kern,
                                                      const float2 *__restrict__ tw, int d0, int d1) {
   using G = RC<N>;
@@ -621,9 +621,11 @@ __global__ __launch_bounds__(FFT_T) void k_c2c_s(float2 *__restrict__ a, const f
    (x',y) line: forward FFT along z, then for d: out_d = IFFT_z(i K_d * .).
    kern: blocked octant storage of (N/2+1)^3 per component (octant=1, odd in
    the d-th wave number, cube_kernf_index) or full storage [d][kz][ky][kx] of
-   N*N*NH (octant=0). */
+   N*N*NH (octant=0).  With one component, out may be spec itself: a
+   workgroup reads its lines whole before it writes them (k_green_r
+   likewise). */
 template <int N>
-__global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ spec, float2 *__restrict__ outd,
+__global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *spec, float2 *outd,
                                                  const float *__restrict__ kern, int octant,
                                                  const float2 *__restrict__ tw, int d0, int d1) {
   constexpr int B = Cfg<N>::BS, TL = FFT_T / B, NP = lspan(N), NH = N / 2 + 1, H = N / 2, SP = cube_spec_pitch(N);
@@ -679,6 +681,29 @@ __global__ __launch_bounds__(FFT_T) void k_green_z(const float2 *__restrict__ sp
       const int l = idx % B, n = idx / B;
       if (col0 + l < NH) dst[n * stride + l] = sm[l * NP + lpad(n)];
     }
+  }
+}
+
+/* Z[k] = E + i O of rows row0..row0+B-1 of a half spectrum into the line
+   buffers sm[l*LP + lpad(k)], k < N/2 (zero past nrows):
+   E = X[k]+conj(X[L-k]), O = (X[k]-conj(X[L-k])) W^-k */
+template <int N, int B>
+__device__ __forceinline__ void c2r_merge_rows(const float2 *spec, float2 *sm, const float2 *__restrict__ tw,
+                                               int64_t row0, int64_t nrows, int tid) {
+  constexpr int L = N / 2, LP = lspan(L);
+  for (int idx = tid; idx < B * L; idx += FFT_T) {
+    const int l = idx / L, k = idx % L;
+    float2 z = make_float2(0.f, 0.f);
+    if (row0 + l < nrows) {
+      const float2 a = spec[(row0 + l) * cube_spec_pitch(N) + k];
+      const float2 cc = spec[(row0 + l) * cube_spec_pitch(N) + (L - k)];
+      float2 w = tw[k];
+      w.y = -w.y;
+      const float2 e = make_float2(a.x + cc.x, a.y - cc.y);
+      const float2 o = cmul(make_float2(a.x - cc.x, a.y + cc.y), w);
+      z = make_float2(e.x - o.y, e.y + o.x);
+    }
+    sm[l * LP + lpad(k)] = z;
   }
 }
 
@@ -793,44 +818,30 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_x3(const float2 *__restrict__ spe
   if (tid == 0) atomicMax(cube_slot(f2slots), (unsigned long long)__float_as_uint(red[0]));
 }
 
-/* One force component of k_c2r_x3, for the compact fine mesh (one force
-   spectrum at a time): component d of the 3-float cells, /div three times.
-   Components 0 and 1 are stored as F; the pass of component 2 (last=1)
-   reads them back, forms max |F|^2 as k_c2r_x3 does and, with pre, stores
-   all three cells' kick factors (((F*a_mid)*dt)/6)/pi.  Every value goes
-   through the same line FFT and divisions as in k_c2r_x3, so the cells end
-   bit-identical to it. */
+/* The compact fine mesh (one force spectrum at a time) forms the three
+   components with the line FFTs and divisions of k_c2r_x3, so its cells end
+   bit-identical to it, but stores components 0 and 1 as planes of their own
+   (whole rows, no partial cache lines): k_c2r_xp writes one component's real
+   rows, /div three times, at pitch opitch floats -- component 0 into the
+   first third of the force mesh, component 1 in place over its own spectrum
+   (each workgroup reads its rows whole before it writes them).  k_c2r_xf
+   then transforms component 2 and writes the 3-float cells (with pre, the
+   kick factors (((F*a_mid)*dt)/6)/pi) and max |F|^2, reading F0 and F1 back
+   from those planes. */
 template <int N>
-__global__ __launch_bounds__(FFT_T) void k_c2r_xc(const float2 *__restrict__ spec, float *__restrict__ force,
-                                                const float2 *__restrict__ tw, int64_t nrows, float div, int d,
-                                                int last, unsigned long long *__restrict__ f2slots, float pa,
-                                                float pdt, int pre) {
+__global__ __launch_bounds__(FFT_T) void k_c2r_xp(const float2 *spec, float *out, int64_t opitch,
+                                                const float2 *__restrict__ tw, int64_t nrows, float div) {
   /* the rows per workgroup of k_c2r_x3, so the line FFTs are the same */
   constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = lspan(L);
   __shared__ float2 sm[B * LP];
-  __shared__ float red[FFT_T];
   const uint32_t db = __float_as_uint(div);
   const bool pow2 = (db & 0x807FFFFFu) == 0u && (db >> 23) > 0u && (db >> 23) < 200u;
   const float inv = 1.0f / div;
   const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
   const int64_t row0 = (int64_t)blockIdx.x * B;
-  for (int idx = tid; idx < B * L; idx += FFT_T) {
-    const int l = idx / L, k = idx % L;
-    float2 z = make_float2(0.f, 0.f);
-    if (row0 + l < nrows) {
-      const float2 a = spec[(row0 + l) * cube_spec_pitch(N) + k];
-      const float2 cc = spec[(row0 + l) * cube_spec_pitch(N) + (L - k)];
-      float2 w = tw[k];
-      w.y = -w.y;
-      const float2 e = make_float2(a.x + cc.x, a.y - cc.y);
-      const float2 o = cmul(make_float2(a.x - cc.x, a.y + cc.y), w);
-      z = make_float2(e.x - o.y, e.y + o.x);
-    }
-    sm[l * LP + lpad(k)] = z;
-  }
+  c2r_merge_rows<N, B>(spec, sm, tw, row0, nrows, tid);
   __syncthreads();
   line_fft<L, +1, TL>(sm + line * LP, tl, tw, N);
-  float m = 0.f;
   for (int idx = tid; idx < B * N; idx += FFT_T) {
     const int l = idx / N, x = idx % N;
     const float2 z = sm[l * LP + lpad((x >> 1))];
@@ -844,13 +855,60 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_xc(const float2 *__restrict__ spe
       v = v / div;
       v = v / div;
     }
-    if (row0 + l >= nrows) continue;
-    float *cell = force + ((row0 + l) * N + x) * 3;
-    if (!last) {
-      cell[d] = v;
-      continue;
+    if (row0 + l < nrows) out[(row0 + l) * opitch + x] = v;
+  }
+}
+
+/* Rows [rbeg, rend) of the last pass.  F0 lies in the first third of the
+   force mesh, so a row's cells overwrite the F0 of rows 3r..3r+2: the host
+   launches the rows in descending chunks [a, b) with 3a >= b, which only
+   overwrite rows of earlier chunks, and every workgroup holds its own F0
+   and F1 in registers before any of it stores (row 0's cells cover its own
+   F0). */
+template <int N>
+__global__ __launch_bounds__(FFT_T) void k_c2r_xf(const float2 *__restrict__ spec, float *force, const float *F0,
+                                                const float *__restrict__ F1, int64_t p1,
+                                                const float2 *__restrict__ tw, int64_t rbeg, int64_t rend,
+                                                float div, unsigned long long *__restrict__ f2slots, float pa,
+                                                float pdt, int pre) {
+  constexpr int L = N / 2, BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC, TL = FFT_T / B, LP = lspan(L);
+  constexpr int IT = (B * N + FFT_T - 1) / FFT_T;
+  __shared__ float2 sm[B * LP];
+  __shared__ float red[FFT_T];
+  const uint32_t db = __float_as_uint(div);
+  const bool pow2 = (db & 0x807FFFFFu) == 0u && (db >> 23) > 0u && (db >> 23) < 200u;
+  const float inv = 1.0f / div;
+  const int tid = threadIdx.x, line = tid / TL, tl = tid % TL;
+  const int64_t row0 = rbeg + (int64_t)blockIdx.x * B;
+  float f0[IT], f1[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * FFT_T, l = idx / N, x = idx % N;
+    const bool in = ((B * N) % FFT_T == 0 || idx < B * N) && row0 + l < rend;
+    f0[i] = in ? F0[(row0 + l) * N + x] : 0.f;
+    f1[i] = in ? F1[(row0 + l) * p1 + x] : 0.f;
+  }
+  c2r_merge_rows<N, B>(spec, sm, tw, row0, rend, tid);
+  __builtin_amdgcn_s_waitcnt(0); /* every F0 load of the workgroup has returned ... */
+  __syncthreads();               /* ... before any of its stores */
+  line_fft<L, +1, TL>(sm + line * LP, tl, tw, N);
+  float m = 0.f;
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * FFT_T, l = idx / N, x = idx % N;
+    if (((B * N) % FFT_T != 0 && idx >= B * N) || row0 + l >= rend) continue;
+    const float2 z = sm[l * LP + lpad((x >> 1))];
+    float v = (x & 1) ? z.y : z.x;
+    if (pow2) {
+      v = v * inv;
+      v = v * inv;
+      v = v * inv;
+    } else {
+      v = v / div;
+      v = v / div;
+      v = v / div;
     }
-    const float f[3] = {cell[0], cell[1], v};
+    const float f[3] = {f0[i], f1[i], v};
     float o[3] = {f[0], f[1], f[2]};
     if (pre) {
 #pragma unroll
@@ -861,13 +919,13 @@ __global__ __launch_bounds__(FFT_T) void k_c2r_xc(const float2 *__restrict__ spe
         o[q] = ok ? gq : (a / 6.0f) / CUBE_PI_F;
       }
     }
-    cell[0] = o[0];
-    cell[1] = o[1];
-    cell[2] = o[2];
+    struct F3 {
+      float x, y, z;
+    };
+    *reinterpret_cast<F3 *>(force + ((row0 + l) * N + x) * 3) = F3{o[0], o[1], o[2]};
     const float f2 = (f[0] * f[0] + f[1] * f[1]) + f[2] * f[2];
     m = fmaxf(m, f2);
   }
-  if (!last) return;
   red[tid] = m;
   __syncthreads();
   for (int q = FFT_T / 2; q > 0; q >>= 1) {
@@ -1024,13 +1082,28 @@ template <int N> struct Launch {
     CUBE_LAUNCH_CHECK();
     return CUBE_OK;
   }
-  static int c2rc(cube_ctx *c, FFTPlan *p, const float2 *spec, float *force, float div, int d,
-                  unsigned long long *f2, float pa, float pdt, int pre) {
+  /* one component's real rows at pitch opitch floats (out may be spec) */
+  static int c2rp(cube_ctx *c, FFTPlan *p, const float2 *spec, float *out, int64_t opitch, float div) {
     constexpr int BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC;
     const int64_t rows = (int64_t)N * N;
-    hipLaunchKernelGGL(k_c2r_xc<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0,
-                       c->stream, spec, force, p->tw, rows, div, d, d == 2, f2, pa, pdt, pre);
+    hipLaunchKernelGGL(k_c2r_xp<N>, dim3((unsigned)((rows + B - 1) / B)), dim3(FFT_T), 0, c->stream, spec, out,
+                       opitch, p->tw, rows, div);
     CUBE_LAUNCH_CHECK();
+    return CUBE_OK;
+  }
+  /* the last pass, rows in descending chunks [a, b), a = ceil(b/3): a
+     chunk's cells overwrite F0 rows >= 3a >= b only, whose cells are done */
+  static int c2rf(cube_ctx *c, FFTPlan *p, const float2 *spec, float *force, const float *F1, int64_t p1,
+                  float div, unsigned long long *f2, float pa, float pdt, int pre) {
+    constexpr int BC = Cfg<N>::BC, B = BC >= 4 ? BC / 2 : BC;
+    for (int64_t b = (int64_t)N * N; b > 0;) {
+      int64_t a = (b + 2) / 3;
+      if (a >= b) a = 0;
+      hipLaunchKernelGGL(k_c2r_xf<N>, dim3((unsigned)((b - a + B - 1) / B)), dim3(FFT_T), 0, c->stream, spec,
+                         force, force, F1, p1, p->tw, a, b, div, f2, pa, pdt, pre);
+      CUBE_LAUNCH_CHECK();
+      b = a;
+    }
     return CUBE_OK;
   }
   static int c2r1(cube_ctx *c, FFTPlan *p, const float2 *spec, float *out) {
@@ -1097,9 +1170,11 @@ int cube_fft_inverse_full(cube_ctx *c, FFTPlan *p, float2 *spec, float *out) {
    three force components (3 floats per cell) and max |F|^2; with prescale
    = {a_mid, dt} the cells hold the fine kick's (((F*a_mid)*dt)/6)/pi
    instead (max |F|^2 still of F).  work3 holds ncomp_spec spectra: 3 runs
-   the three components together, 1 one after the other through the same
-   spectrum (the compact fine mesh), with bit-identical results. */
-int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
+   the three components together, 1 one after the other (the compact fine
+   mesh), with bit-identical results: components 0 and 1 through work3 into
+   planes (the first third of force3, then work3 itself), component 2
+   through spec in place, so spec does not survive the call. */
+int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, float2 *spec, const float *kern, int kern_octant,
                           float2 *work3, float *force3, float div, float *f2max, int t0, unsigned *f2dev,
                           unsigned long long *f2slots, const float *prescale, int ncomp_spec) {
   const float pa = prescale ? prescale[0] : 0.f, pdt = prescale ? prescale[1] : 0.f;
@@ -1121,13 +1196,21 @@ int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const flo
       if ((st = Launch<N>::c2r3(c, p, work3, force3, div, f2slots, pa, pdt, prescale != nullptr))) return st; \
       if (t0 >= 0) tend(c, t0 + 2);                                               \
     } else {                                                                      \
+      /* component 2 goes through spec in place (its last reader) */              \
       for (int d = 0; d < 3; ++d) {                                               \
+        float2 *sd = d < 2 ? work3 : spec;                                        \
         if (t0 >= 0) tbeg(c);                                                     \
-        if ((st = Launch<N>::green(c, p, spec, kern, kern_octant, work3, d, d + 1))) return st; \
+        if ((st = Launch<N>::green(c, p, spec, kern, kern_octant, sd, d, d + 1))) return st; \
         if (t0 >= 0) { tend(c, t0); tbeg(c); }                                    \
-        if ((st = Launch<N>::c2c_y<+1>(c, p, work3, 1))) return st;               \
+        if ((st = Launch<N>::c2c_y<+1>(c, p, sd, 1))) return st;                  \
         if (t0 >= 0) { tend(c, t0 + 1); tbeg(c); }                                \
-        if ((st = Launch<N>::c2rc(c, p, work3, force3, div, d, f2slots, pa, pdt, prescale != nullptr))) return st; \
+        if (d == 0) st = Launch<N>::c2rp(c, p, sd, force3, N, div);               \
+        else if (d == 1)                                                          \
+          st = Launch<N>::c2rp(c, p, sd, reinterpret_cast<float *>(work3), 2 * (int64_t)cube_spec_pitch(N), div); \
+        else                                                                      \
+          st = Launch<N>::c2rf(c, p, sd, force3, reinterpret_cast<const float *>(work3), 2 * (int64_t)cube_spec_pitch(N), \
+                               div, f2slots, pa, pdt, prescale != nullptr);       \
+        if (st) return st;                                                        \
         if (t0 >= 0) tend(c, t0 + 2);                                             \
       }                                                                           \
     }                                                                             \

@@ -293,7 +293,7 @@ void cube_fft_twiddles(int n, float2 *h);
 int cube_fft_create(cube_ctx *c, int n, FFTPlan **out);
 void cube_fft_destroy(FFTPlan *p);
 int cube_fft_r2c_3d(cube_ctx *c, FFTPlan *p, const float *in, float2 *out);
-int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, const float2 *spec, const float *kern, int kern_octant,
+int cube_fft_green_c2r_3d(cube_ctx *c, FFTPlan *p, float2 *spec, const float *kern, int kern_octant,
                           float2 *work3, float *force3, float scale_div, float *f2max, int timer0,
                           unsigned *f2dev, unsigned long long *f2slots, const float *prescale = nullptr,
                           int ncomp_spec = 3);
