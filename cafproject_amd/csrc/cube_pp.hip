@@ -208,6 +208,7 @@ __global__ __launch_bounds__(PT) void k_pp(const typename IntOf<IX>::T *__restri
  * `dlist` with its number of work items in `ditems`.
  */
 #define PPB_T 256
+#define PP_HEAVY 32 /* default heavy_at of the block kernel: staged cells above it are placed by their wave */
 #define PPC_T 256
 __device__ static inline int pp_fix(int IX, int cc, int32_t code) {
   return (cc - 1) * (4 * 32768) + (int)(2 * cube_ucode(IX, code) + 1) * (IX == 1 ? 256 : 1);
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
                                                   double rsoft, double mass, double a_mid, double dt, double venc,
                                                   int maxs, int fdiv, unsigned long long *__restrict__ out,
                                                   int32_t *__restrict__ dlist, int32_t *__restrict__ ditems,
-                                                  unsigned long long *__restrict__ dcount) {
+                                                  unsigned long long *__restrict__ dcount, int heavy_at) {
   extern __shared__ uint64_t smem64[];
   const int RF = 16 + 2 * pr, nreg = RF * RF * RF;
   uint64_t *pos = smem64;                                            /* packed positions (maxs) */
@@ -318,16 +319,68 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   }
   __syncthreads();
   if (np > 65535) wide = 1;
-  /* the region cell of a particle of this thread's staged cell, or -1 */
-  auto region = [&](const int32_t code[3]) -> int {
+  /* the region cell of a particle of staged cell c, or -1 */
+  auto region_in = [&](const int c[3], const int32_t code[3]) -> int {
     bool in = true;
     int r[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      r[a] = 4 * (sc[a] - 1) + (int)(cube_ucode(IX, code[a]) >> sh) + 1 - flo[a];
+      r[a] = 4 * (c[a] - 1) + (int)(cube_ucode(IX, code[a]) >> sh) + 1 - flo[a];
       in = in && r[a] >= 0 && r[a] < RF;
     }
     return in ? (r[2] * RF + r[1]) * RF + r[0] : -1;
+  };
+  auto region = [&](const int32_t code[3]) -> int { return region_in(sc, code); };
+  /* A staged cell of more than heavy_at particles would hold its wave for
+     its whole length: its thread hands it to the wave, which counts and
+     places 64 of its particles at a time after the light cells.  Lanes of
+     one fine cell are matched by six ballots on the fine sub-cell index; a
+     lane's rank among the lower lanes of its fine cell orders it as the
+     serial walk does (the walk that places runs down l, lane 0 first). */
+  const bool heavy = np > heavy_at;
+  const int npl = heavy ? 0 : np;
+  auto heavy_cells = [&](bool place) {
+    const int lane = tid & 63;
+    uint64_t hm = __ballot(heavy);
+    while (hm) {
+      const int src = __ffsll((unsigned long long)hm) - 1;
+      hm &= hm - 1;
+      const int hnp = __shfl(np, src, 64);
+      const int64_t hp0 = (int64_t)(((uint64_t)(uint32_t)__shfl((int)(p0 >> 32), src, 64) << 32) |
+                                    (uint64_t)(uint32_t)__shfl((int)p0, src, 64));
+      const int hsc[3] = {__shfl(sc[0], src, 64), __shfl(sc[1], src, 64), __shfl(sc[2], src, 64)};
+      for (int k = 0; 64 * k < hnp; ++k) {
+        const int l = place ? hnp - 1 - 64 * k - lane : 64 * k + lane;
+        int32_t code[3] = {0, 0, 0};
+        int ri = -1;
+        if (l >= 0 && l < hnp) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) code[a] = (int32_t)xp[3 * (hp0 + l) + a];
+          ri = region_in(hsc, code);
+        }
+        const int sub = (int)((cube_ucode(IX, code[2]) >> sh) << 4 | (cube_ucode(IX, code[1]) >> sh) << 2 |
+                              (cube_ucode(IX, code[0]) >> sh));
+        uint64_t m = __ballot(ri >= 0);
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+          const uint64_t bb = __ballot((sub >> b) & 1);
+          m &= ((sub >> b) & 1) ? bb : ~bb;
+        }
+        if (ri >= 0) {
+          const int rank = __popcll(m & ((1ull << lane) - 1ull));
+          if (!place) {
+            if (rank == 0) cur[ri] = (uint16_t)(cur[ri] + __popcll(m));
+          } else {
+            const int slot = cur[ri] + rank;
+            int r[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) r[a] = pp_fix(IX, hsc[a], code[a]) - X0[a];
+            pos[slot] = pp_pack(r);
+            if ((m >> lane) == 1ull) cur[ri] = (uint16_t)(slot + 1); /* the last lane of its fine cell */
+          }
+        }
+      }
+    }
   };
   /* the codes of particles l0, l0+d, l0+2d, l0+3d (clamped to [0, np)) in
      flight together */
@@ -339,17 +392,18 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
       for (int a = 0; a < 3; ++a) c4[u][a] = (int32_t)xp[3 * (p0 + l) + a];
     }
   };
-  /* count per fine cell: the cell's owner thread only, no atomics */
-  for (int l0 = 0; l0 < np; l0 += 4) {
+  /* count per fine cell: the cell's owner thread (or wave) only, no atomics */
+  for (int l0 = 0; l0 < npl; l0 += 4) {
     int32_t c4[4][3];
     codes4(l0, 1, c4);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      if (l0 + u >= np) break;
+      if (l0 + u >= npl) break;
       const int ri = region(c4[u]);
       if (ri >= 0) cur[ri] = (uint16_t)(cur[ri] + 1);
     }
   }
+  heavy_cells(false);
   __syncthreads();
   /* exclusive scan of the counts */
   const int chunk = (nreg + PPB_T - 1) / PPB_T;
@@ -381,7 +435,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
   __syncthreads();
   /* chain order inside a fine cell is decreasing l: walk l downwards and
      append; afterwards cur[f] is the end of fine cell f's list */
-  for (int l0 = np - 1; l0 >= 0; l0 -= 4) {
+  for (int l0 = npl - 1; l0 >= 0; l0 -= 4) {
     int32_t c4[4][3];
     codes4(l0, -1, c4);
 #pragma unroll
@@ -397,6 +451,7 @@ __global__ __launch_bounds__(PPB_T) void k_pp_blk(const typename IntOf<IX>::T *_
       pos[slot] = pp_pack(r);
     }
   }
+  heavy_cells(true);
   if (tid == 0) {
     int acc = 0;
     for (int q2 = 0; q2 < 16; ++q2) {
@@ -760,6 +815,8 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
     if (hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
       ldsmax = 65536;
     if (const char *e = getenv("CUBE_PP_STAGE_MAX"); e && *e) maxs = std::max(0, std::min(maxs, atoi(e)));
+    int heavy_at = PP_HEAVY;
+    if (const char *e = getenv("CUBE_PP_HEAVY"); e && *e) heavy_at = std::max(0, atoi(e));
     if (const char *e = getenv("CUBE_PP_CELL_CAP"); e && *e)
       cap = std::max(1, std::min((ldsmax - 1024 - (2 * nregc + 1) * 4) / 8, atoi(e)));
     const int64_t nb = c->nt / 4, nblk = nb * nb * nb * c->ntiles;
@@ -769,7 +826,7 @@ static int pp_impl(cube_ctx *c, float a_mid, float dt, float sigma_vi, float mas
     hipLaunchKernelGGL((k_pp_blk<IX, IV>), dim3((unsigned)nblk), dim3(PPB_T), (size_t)maxs * 8 + table, c->stream,
                        (const typename IntOf<IX>::T *)c->xp, (typename IntOf<IV>::T *)c->vp, c->rhoc, c->cstart,
                        c->vdec0, c->venc_tab, geo_of(c), c->nvbin / 2, pr, (double)c->p.rsoft, (double)mass_p, (double)a_mid,
-                       (double)dt, cube_vconst(sigma_vi), maxs, fdiv, out, c->ppdefer, c->ppitems, dcount);
+                       (double)dt, cube_vconst(sigma_vi), maxs, fdiv, out, c->ppdefer, c->ppitems, dcount, heavy_at);
     CUBE_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pp_item_scan, dim3(1), dim3(1024), 0, c->stream, c->ppitems, dcount);
     CUBE_LAUNCH_CHECK();

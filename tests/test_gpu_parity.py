@@ -245,18 +245,24 @@ def test_deposits_clustered_bit_exact(heavy, monkeypatch):
     assert np.array_equal(g.get_field("rho_f"), global_from_tiles(tiles, nc, nnt, key="rho"))
 
 
-@pytest.mark.parametrize("stage_max,cell_cap", [(None, None), ("0", None), (None, "300")])
-def test_pp_clustered_bit_exact(stage_max, cell_cap, monkeypatch):
+@pytest.mark.parametrize("stage_max,cell_cap,heavy",
+                         [(None, None, None), ("0", None, None), (None, "300", None), (None, None, "0"),
+                          (None, None, "16")])
+def test_pp_clustered_bit_exact(stage_max, cell_cap, heavy, monkeypatch):
     """A collapsed clump (6000 particles, Gaussian with sigma 0.7 coarse
     cells) in a uniform background of 10000: the blocks around it go to the
     cell path, its coarse cells hold several 256-particle items each, and
-    their neighbourhoods hold more than one staging window.  Velocities and
-    max |f|^2 match the oracle bit for bit."""
+    their neighbourhoods hold more than one staging window; the block
+    kernel's staged cells above CUBE_PP_HEAVY particles are counted and
+    placed by their whole wave (0: every cell).  Velocities and max |f|^2
+    match the oracle bit for bit."""
     from cafproject_amd.cube import Cube
     if stage_max is not None:
         monkeypatch.setenv("CUBE_PP_STAGE_MAX", stage_max)
     if cell_cap is not None:
         monkeypatch.setenv("CUBE_PP_CELL_CAP", cell_cap)
+    if heavy is not None:
+        monkeypatch.setenv("CUBE_PP_HEAVY", heavy)
     nc, nnt = 16, 1
     pos, vel = _clump_state(nc)
     sig = 0.3
@@ -271,6 +277,22 @@ def test_pp_clustered_bit_exact(stage_max, cell_cap, monkeypatch):
     m = float(np.float32((4 * nc) ** 3 / 16000))
     f_o = oc.ext_pp_force(0.05, 2.0, sig, m)
     f_g = g.ext_pp_force(0.05, 2.0, sig, m)
+    _same_state(g.get_state(), oc.get_state())
+    assert f_g == f_o
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+@pytest.mark.parametrize("heavy", ["0", "6"])
+def test_pp_wave_staging_bit_exact(case, heavy, monkeypatch):
+    """The block kernel's staged coarse cells above CUBE_PP_HEAVY particles
+    are counted and placed by their whole wave, 64 particles at a time, in
+    the chain order of the one-thread walk: with every cell (0) or the
+    fuller ones (6) on that path the forces match the oracle bit for bit."""
+    monkeypatch.setenv("CUBE_PP_HEAVY", heavy)
+    st, oc, g = _pair(case, pp_on=True)
+    m = st["mass_p"]
+    f_o = oc.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
+    f_g = g.ext_pp_force(0.02, 3.0, st["sigma_vi"], m)
     _same_state(g.get_state(), oc.get_state())
     assert f_g == f_o
 
