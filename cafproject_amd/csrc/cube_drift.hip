@@ -577,21 +577,34 @@ __global__ __launch_bounds__(256) void k_drift_final2(
   const bool one = flag[blockIdx.x] != 0;
   if (one) {
     for (int64_t s = S0 + tid; s < S1; s += 256) pix[s - S0] = keys[s];
-    __syncthreads();
     if (tid < ncl) {
-      /* insertion sort of the cell's keys (unique: they carry the particle index) */
       const int lo = (int)(cst[tid] - S0), hi = (int)(cst[tid + 1] - S0);
       for (int q = lo; q < hi; ++q) lcell[q] = (uint8_t)tid;
-      for (int q = lo + 1; q < hi; ++q) {
+    }
+    __syncthreads();
+    /* each key's place in its cell is the number of smaller keys there
+       (unique: they carry the particle index); every lane ranks its own
+       slots at once instead of one lane per cell sorting serially */
+    constexpr int PER = (FB_CH + 255) / 256;
+    uint64_t kx[PER];
+    int dst[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int q = tid + 256 * u;
+      dst[u] = -1;
+      if (q < (int)(S1 - S0)) {
+        const int c = lcell[q], lo = (int)(cst[c] - S0), hi = (int)(cst[c + 1] - S0);
         const uint64_t x = pix[q];
-        int r = q - 1;
-        while (r >= lo && pix[r] > x) {
-          pix[r + 1] = pix[r];
-          --r;
-        }
-        pix[r + 1] = x;
+        int r = 0;
+        for (int w = lo; w < hi; ++w) r += pix[w] < x ? 1 : 0;
+        kx[u] = x;
+        dst[u] = lo + r;
       }
     }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+      if (dst[u] >= 0) pix[dst[u]] = kx[u];
     __syncthreads();
   }
   auto cell_of = [&](int64_t s) -> int {
