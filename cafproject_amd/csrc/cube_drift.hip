@@ -225,13 +225,22 @@ __global__ __launch_bounds__(DT) void k_drift_dest(const typename IntOf<IX>::T *
   __syncthreads();
   const int64_t pbeg = cs[0], pend = cs[nt];
   const bool agg = pend - pbeg <= P.agg_cap;   /* uniform over the block */
+  if (agg) { /* pslot first holds each particle's cell of the row */
+    for (int c = threadIdx.x; c < nt; c += DT)
+      for (int64_t q = cs[c]; q < cs[c + 1]; ++q) pslot[q - pbeg] = (uint32_t)c;
+    __syncthreads();
+  }
   for (int64_t p = pbeg + threadIdx.x; p < pend; p += DT) {
-    if (agg) pslot[p - pbeg] = ~0u;
     /* cell of p: largest i with cs[i] <= p */
     int lo = 0, hi = nt;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (cs[mid] <= p) lo = mid; else hi = mid;
+    if (agg) {
+      lo = (int)pslot[p - pbeg];
+      pslot[p - pbeg] = ~0u;
+    } else {
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cs[mid] <= p) lo = mid; else hi = mid;
+      }
     }
     const int i = lo;
     const int home[3] = {i + 1, j + 1, k + 1};   /* 1-based tile-local */

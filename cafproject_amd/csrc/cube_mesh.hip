@@ -788,6 +788,7 @@ __device__ __forceinline__ void vmax_reduce(double vm[3], unsigned long long *__
    re-decoded code (vdec2, venc2) in the same pass, which is the step's
    sequence W/pm.f90:129-159 then 322-354 with the intermediate codes kept
    in registers instead of going through vp. */
+#define KICK_ROW 2048 /* particles of a row whose cells the kick tables in LDS (else a binary search) */
 template <int IX, int IV, bool FINE, int NKICK>
 __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__restrict__ xp,
                                              typename IntOf<IV>::T *__restrict__ vp,
@@ -802,16 +803,27 @@ __global__ __launch_bounds__(KT) void k_kick(const typename IntOf<IX>::T *__rest
   const int j = (int)(row % nt), k = (int)((row / nt) % nt), t = (int)(row / ((int64_t)nt * nt));
   const int tz = t / (g.nnt * g.nnt), ty = (t / g.nnt) % g.nnt, tx = t % g.nnt;
   __shared__ int64_t cs[520];
+  __shared__ uint16_t lcell[KICK_ROW]; /* slot -> cell of the row, when it fits */
   const int64_t c0 = row * nt;
   for (int i = threadIdx.x; i <= nt; i += KT) cs[i] = cstart[c0 + i];
   __syncthreads();
+  const bool tab = cs[nt] - cs[0] <= KICK_ROW; /* uniform */
+  if (tab) {
+    for (int i = threadIdx.x; i < nt; i += KT)
+      for (int64_t q = cs[i]; q < cs[i + 1]; ++q) lcell[q - cs[0]] = (uint16_t)i;
+    __syncthreads();
+  }
   constexpr bool VMAX = !FINE || NKICK == 2; /* the coarse kick reports vmax */
   double vm[3] = {0, 0, 0};
   for (int64_t p = cs[0] + threadIdx.x; p < cs[nt]; p += KT) {
     int lo = 0, hi = nt;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (cs[mid] <= p) lo = mid; else hi = mid;
+    if (tab) {
+      lo = lcell[p - cs[0]];
+    } else {
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cs[mid] <= p) lo = mid; else hi = mid;
+      }
     }
     const int loc[3] = {lo + 1, j + 1, k + 1};
     const int tt[3] = {tx, ty, tz};
